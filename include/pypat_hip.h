@@ -1,0 +1,225 @@
+/*
+ * pypat_hip.h -- C ABI of the MI355X particle push + field gather engine
+ * (libpypat_hip.so).
+ *
+ * The reference (PyPATools) is pure Python; its only "native" functions are the
+ * Numba-JIT batch kernels and the Pusher methods that drive them.  Each entry
+ * point below names the reference function it replaces (file:line, relative to
+ * the PyPATools checkout).
+ *
+ * Conventions
+ *   - Every function returns 0 (PP_OK) or a PP_ERR_* code; pp_last_error()
+ *     gives the message of the last failure on the calling thread.  No C++
+ *     exception crosses this boundary.
+ *   - Compute entry points take DEVICE pointers (allocate with pp_malloc, move
+ *     data with pp_memcpy_*) and a stream handle (NULL = the library's default
+ *     stream).  They enqueue work and return without synchronising unless
+ *     documented as blocking.
+ *   - Particle state is SoA float64: x, y, z [m] and vx, vy, vz.  For the
+ *     u-state algorithms (PP_ALG_BORIS_REL, PP_ALG_VAY) vx..vz hold
+ *     u = gamma*v [m/s]; convert with pp_v_to_u / pp_u_to_v.
+ *   - Point and field arrays at the gather boundary are float64 (M, 3) C-order,
+ *     as Field.__call__ takes and returns them (field.py:213-244).
+ *   - The library owns pp_map objects and frees them in pp_map_destroy.
+ */
+#ifndef PYPAT_HIP_H
+#define PYPAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_ABI_VERSION 1
+
+enum pp_status {
+    PP_OK = 0,
+    PP_ERR_INVALID = 1,      /* bad argument (ValueError on the Python side) */
+    PP_ERR_HIP = 2,          /* HIP runtime failure */
+    PP_ERR_NOMEM = 3,        /* device allocation refused */
+    PP_ERR_UNSUPPORTED = 4   /* valid request this build does not implement */
+};
+
+/* Integrators, Pusher.ALGORITHMS (pusher.py:465-466) + two extensions. */
+enum pp_algorithm {
+    PP_ALG_LEAPFROG = 0,     /* pusher.py:113-121, 730-745 */
+    PP_ALG_BORIS = 1,        /* pusher.py:124-165, 369-379 (non-relativistic) */
+    PP_ALG_RK4 = 2,          /* pusher.py:803-892 */
+    PP_ALG_YOSHIDA = 3,      /* pusher.py:184-209 */
+    PP_ALG_VAY_REL = 4,      /* pusher.py:216-279, 382-392 */
+    PP_ALG_RK4_REL = 5,      /* pusher.py:282-329, 803-892 */
+    PP_ALG_YOSHIDA_REL = 6,  /* pusher.py:332-362 */
+    PP_ALG_BORIS_REL = 7,    /* extension: relativistic Boris on u = gamma v */
+    PP_ALG_VAY = 8           /* extension: Vay, Phys. Plasmas 15, 056701 (2008) */
+};
+
+/* Arithmetic of a launch. */
+enum pp_precision {
+    PP_PREC_FP64 = 0,        /* IEEE FP64 in the reference's NumPy operation order, no FMA
+                                contraction: bitwise equal to the FP64 oracle */
+    PP_PREC_FP64_FAST = 1,   /* FP64 with FMA contraction and precomputed cell coefficients */
+    PP_PREC_FP32_MIXED = 2   /* FP32 map + FP32 increments, FP64 position/velocity accumulation */
+};
+
+enum pp_field_kind { PP_FIELD_ZERO = 0, PP_FIELD_CONST = 1, PP_FIELD_MAP = 2, PP_FIELD_ARRAY = 3 };
+enum pp_rf_mode { PP_RF_NONE = 0, PP_RF_COS = 1, PP_RF_SIN = 2 };
+enum pp_map_flags { PP_MAP_PERIODIC_Z = 1, PP_MAP_NO_FP32 = 2, PP_MAP_NO_COEF = 4 };
+
+typedef struct pp_map pp_map;          /* device-resident field map (opaque) */
+typedef void* pp_stream;               /* hipStream_t */
+
+/*
+ * One field term, the device form of a Field / ScaledField (field.py:101-690):
+ *   value = ((scale1 * F) [* scale2] [+ offset]) [* rf(t)]
+ * F is the constant `value` (kind CONST, already multiplied by Field.scaling),
+ * the map components comp0..comp0+2 (kind MAP), or per-particle values from a
+ * device (n, 3) array (kind ARRAY: fields evaluated elsewhere, e.g. by a Python
+ * callable; single-step pushes of single-gather algorithms only).  rf(t) is read
+ * from the launch's rf table (cos or sin column), see pp_push_args.
+ */
+typedef struct pp_field {
+    int32_t kind;            /* enum pp_field_kind */
+    int32_t comp0;           /* first map component used (MAP) */
+    const pp_map* map;       /* MAP */
+    const double* array;     /* ARRAY: device (n, 3) */
+    double scale1;           /* Field.scaling (field.py:500-502) */
+    double scale2;           /* ScaledField.scale (field.py:690), used if has_scale2 */
+    double offset;           /* ScaledField.offset, used if has_offset */
+    int32_t has_scale2;
+    int32_t has_offset;
+    int32_t rf;              /* enum pp_rf_mode */
+    int32_t _pad;
+    double value[3];         /* CONST */
+} pp_field;
+
+/*
+ * A fused multi-step push of n particles (Pusher.track_batch's step loop,
+ * pusher.py:966-990, with push_batch pusher.py:699-801 inside).
+ */
+typedef struct pp_push_args {
+    int32_t algorithm;       /* enum pp_algorithm */
+    int32_t precision;       /* enum pp_precision */
+    int64_t n;               /* particles */
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    uint8_t* alive;          /* optional; 0 = inactive: not pushed, not recorded */
+    const double* dt_pp;     /* optional per-particle dt (RK4 family, pusher.py:815-816) */
+    double dt;               /* [s] */
+    double q_over_m;         /* [C/kg] */
+    int64_t nsteps;          /* steps in this call */
+    int64_t step0;           /* global index of the first step */
+    pp_field efield;
+    pp_field bfield;
+    /* RF table: rf_table[(step - rf_step0) * 8 + stage * 2 + {0: cos, 1: sin}],
+       stage 0..3 at t_n + (0, 1/2, 1/2, 1) dt; required if a field has rf != NONE */
+    const double* rf_table;
+    int64_t rf_step0;
+    int64_t rf_nsteps;       /* rows in rf_table (bounds check) */
+    /* history (optional): device AoS (n_records, n, 3); record k holds the state
+       after global step k*rec_every - 1, k = 1 .. n_records-1 (pusher.py:975-980) */
+    double* r_hist;
+    double* v_hist;
+    int64_t rec_every;
+    int64_t n_records;
+    int32_t kick_only;       /* 1: velocity-only kick with this dt (Boris staggering,
+                                pusher.py:946-958, 992-1007); nsteps must be 1 */
+    int32_t block_size;      /* 0 = library default */
+} pp_push_args;
+
+/* ---- runtime ------------------------------------------------------------ */
+int pp_init(int device);                                   /* select device, create default stream */
+const char* pp_last_error(void);
+int pp_abi_version(void);
+/* The constants the kernels use, as the reference's Python expressions give them:
+   out[0] CLIGHT, out[1] CLIGHT**2 (pusher.py:90), out[2] 0.9999*CLIGHT (:276),
+   out[3..5] YOSHIDA_C1..C3, out[6..8] YOSHIDA_D1..D3 (:176-181).  n >= 9. */
+int pp_get_constants(double* out, int32_t n);
+/* sizeof(pp_field), sizeof(pp_push_args) as compiled (binding layout check). */
+int pp_struct_sizes(int64_t* field_size, int64_t* push_args_size);
+int pp_device_count(int* count);
+int pp_mem_info(size_t* free_bytes, size_t* total_bytes);  /* hipMemGetInfo */
+int pp_malloc(void** dptr, size_t bytes);
+int pp_free(void* dptr);
+int pp_memcpy_h2d(void* dst, const void* src, size_t bytes, pp_stream s);
+int pp_memcpy_d2h(void* dst, const void* src, size_t bytes, pp_stream s);
+int pp_memcpy_d2d(void* dst, const void* src, size_t bytes, pp_stream s);
+int pp_fill_f64(double* dptr, double value, int64_t count, pp_stream s);   /* kernel fill, graph-safe */
+int pp_stream_create(pp_stream* s);
+int pp_stream_destroy(pp_stream s);
+int pp_stream_sync(pp_stream s);
+int pp_device_sync(void);
+/* events for timing: pp_event is a hipEvent_t */
+typedef void* pp_event;
+int pp_event_create(pp_event* e);
+int pp_event_destroy(pp_event e);
+int pp_event_record(pp_event e, pp_stream s);
+int pp_event_elapsed_ms(float* ms, pp_event start, pp_event stop);   /* blocking on stop */
+
+/* ---- field maps (Field.from_arrays + get_interpolator, field.py:345-387,
+        interpolators.py:533-610) ------------------------------------------ */
+/*
+ * values: host float64 [ncomp][nx][ny][nz] (each component 'ij'-ordered, z
+ * fastest, as NumbaInterpolator stores it, interpolators.py:314-315).
+ * gx, gy, gz: host node coordinates, strictly increasing; an axis with one node
+ * is inactive (1-D fields use the z axis, 2-D fields x and y, as
+ * Field._get_field_1d/_2d evaluate them, field.py:457-489).
+ * fill_value: value of every component outside the grid (bounds_error=False,
+ * interpolators.py:94-97; Field uses 0.0, field.py:380-383).
+ * flags: PP_MAP_PERIODIC_Z wraps z with period nz*dz (extension).
+ * Blocking (uploads and builds the cell-coefficient tables).
+ */
+int pp_map_create(const double* values, int32_t ncomp, int32_t nx, int32_t ny, int32_t nz,
+                  const double* gx, const double* gy, const double* gz, double fill_value, int32_t flags,
+                  pp_map** out);
+int pp_map_destroy(pp_map* map);
+int pp_map_bytes(const pp_map* map, size_t* bytes);
+
+/* Field.__call__ for one field term at M points (field.py:213-244 -> _get_field_0d/1d/2d/3d
+   -> NumbaInterpolator._interp3d_batch, interpolators.py:259-267).
+   pts, out: device float64 (M, 3).  rf terms are evaluated with factor rf_value. */
+int pp_field_eval(const pp_field* field, int32_t precision, const double* pts, int64_t m,
+                  double rf_value, double* out, pp_stream s);
+
+/* ---- pushers -------------------------------------------------------------- */
+/* Fused multi-step push: Pusher.track_batch step loop (pusher.py:966-990) /
+   push_batch (pusher.py:699-801) when nsteps == 1. */
+int pp_push(const pp_push_args* args, pp_stream s);
+
+/* Per-step kernels with fields supplied per particle (the reference's Numba batch
+   kernels).  v, E, B, r and outputs: device float64 (M, 3). */
+int pp_boris_push_batch(const double* v, const double* E, const double* B, int64_t m,
+                        double dt, double q_over_m, double* v_out, pp_stream s);          /* pusher.py:369-379 */
+int pp_vay_push_batch(const double* v, const double* E, const double* B, int64_t m,
+                      double dt, double q_over_m, double* v_out, pp_stream s);            /* pusher.py:382-392 */
+int pp_rk4_rel_dbetagamma_dt_batch(const double* v, const double* E, const double* B, int64_t m,
+                                   double q_over_m, double* out, pp_stream s);            /* pusher.py:301-329 */
+int pp_position_update_batch(const double* r, const double* v, int64_t m, double dt,
+                             double* r_out, pp_stream s);                                  /* pusher.py:395-404 */
+
+/* Elementwise stage kernels on (M,3) arrays, one IEEE operation per NumPy
+   operation of _rk4_step_batch (pusher.py:818-890) and the field algebra
+   (field.py:641-653, 689-690).  s_pp / w_pp: optional per-particle factors
+   replacing the scalar. */
+int pp_lorentz_batch(const double* v, const double* E, const double* B, int64_t m, double q_over_m,
+                     double* out, pp_stream s);                         /* q/m (E + v x B), pusher.py:828-829 */
+int pp_axpy_batch(const double* a, double s_, const double* s_pp, const double* b, int64_t m,
+                  double* out, pp_stream s);                            /* a + s*b */
+int pp_rk4_combine_batch(const double* a, double w, const double* w_pp, const double* k1, const double* k2,
+                         const double* k3, const double* k4, int64_t m, double* out, pp_stream s);
+                                                                        /* a + w*(((k1 + 2k2) + 2k3) + k4) */
+int pp_clamp_speed_batch(const double* v, int64_t m, double* out, pp_stream s);   /* pusher.py:886-890 */
+int pp_scale_offset_batch(const double* a, double scale, int32_t has_scale, double offset, int32_t has_offset,
+                          int64_t m, double* out, pp_stream s);          /* scale*a + offset, field.py:690 */
+
+/* AoS (M,3) <-> SoA and v <-> u conversions (state staging for pp_push). */
+int pp_aos_to_soa(const double* aos, int64_t m, double* a, double* b, double* c, pp_stream s);
+int pp_soa_to_aos(const double* a, const double* b, const double* c, int64_t m, double* aos, pp_stream s);
+int pp_v_to_u(double* vx, double* vy, double* vz, int64_t m, pp_stream s);   /* u = gamma(v) v, pusher.py:74-96 clamp */
+int pp_u_to_v(double* ux, double* uy, double* uz, int64_t m, pp_stream s);   /* v = u / sqrt(1 + u^2/c^2) */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYPAT_HIP_H */

@@ -1,0 +1,68 @@
+"""Build libpypat_hip.so in-tree for gfx950 with hipcc.
+
+    python -m pypatools_amd.build            # or __graft_entry__.build()
+
+Each translation unit is compiled on its own (the strict TU must not see FMA
+contraction), then linked into pypatools_amd/libpypat_hip.so.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+OUT = os.path.join(PKG, "libpypat_hip.so")
+BUILD_DIR = os.path.join(PKG, "_build")
+
+ARCH = os.environ.get("PYPAT_OFFLOAD_ARCH", "gfx950")
+SOURCES = ["pp_host.hip", "pp_strict.hip", "pp_fast.hip"]
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-I", INCLUDE, "-I", CSRC,
+    # packed FP32 (v_pk_*_f32) stays off: scalar FP32 only
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+PER_FILE = {
+    "pp_strict.hip": ["-ffp-contract=off"],
+}
+
+
+def hipcc():
+    exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(exe):
+        raise RuntimeError("hipcc not found; the HIP library cannot be built")
+    return exe
+
+
+def _compile(src):
+    obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
+    cmd = [hipcc(), "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + PER_FILE.get(src, [])
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr[-6000:]}")
+    return obj
+
+
+def build(verbose=True):
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{res.stderr[-6000:]}")
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT} ({os.path.getsize(OUT) // 1024} KiB) for {ARCH}")
+    return OUT
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,598 @@
+// Host runtime of libpypat_hip.so: the C ABI declared in include/pypat_hip.h.
+// Validation happens here, on the host, before any launch: every kernel can then
+// assume its indices and table rows are in range.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "pp_internal.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+std::mutex g_mu;
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+
+// Constants from the reference's Python expressions (global_variables.py:36,
+// pusher.py:90, :276, :176-181), evaluated identically in IEEE double.
+const double kClight = 299792458.0;
+// YOSHIDA_C1..C3, D1..D3 as the Python expressions of pusher.py:176-181 evaluate
+// them (1/(2 - 2^(1/3)), -2^(1/3)/(2 - 2^(1/3)), ...), written out exactly.
+const double kYoshidaC[3] = {0x1.59e8b6eb96339p+0, -0x1.b3d16dd72c671p+0, 0x1.59e8b6eb96339p+0};
+const double kYoshidaD[3] = {0x1.59e8b6eb96339p+1, -0x1.86dd1261614d6p+1, 0x1.59e8b6eb96339p+1};
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? PP_ERR_NOMEM : PP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                        \
+    do {                                           \
+        hipError_t _e = (expr);                    \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+hipStream_t stream_of(pp_stream s) { return s ? static_cast<hipStream_t>(s) : g_stream; }
+
+int ensure_init() {
+    if (g_device >= 0) return PP_OK;
+    return pp_init(0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_abi_version(void) { return PP_ABI_VERSION; }
+
+int pp_struct_sizes(int64_t* field_size, int64_t* push_args_size) {
+    if (!field_size || !push_args_size) return fail(PP_ERR_INVALID, "null argument");
+    *field_size = (int64_t)sizeof(pp_field);
+    *push_args_size = (int64_t)sizeof(pp_push_args);
+    return PP_OK;
+}
+
+int pp_get_constants(double* out, int32_t n) {
+    if (!out || n < 9) return fail(PP_ERR_INVALID, "need room for 9 constants");
+    out[0] = kClight;
+    out[1] = kClight * kClight;
+    out[2] = 0.9999 * kClight;
+    for (int k = 0; k < 3; ++k) {
+        out[3 + k] = kYoshidaC[k];
+        out[6 + k] = kYoshidaD[k];
+    }
+    return PP_OK;
+}
+
+const char* pp_last_error(void) { return g_err; }
+
+int pp_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count) return fail(PP_ERR_INVALID, "device %d out of range (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    if (g_device != device) {
+        if (g_stream) hipStreamDestroy(g_stream);
+        g_stream = nullptr;
+        // default priority (no priority streams on this engine)
+        HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
+        g_device = device;
+    }
+    return PP_OK;
+}
+
+int pp_device_count(int* count) {
+    if (!count) return fail(PP_ERR_INVALID, "null count");
+    HIP_TRY(hipGetDeviceCount(count), "hipGetDeviceCount");
+    return PP_OK;
+}
+
+int pp_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    HIP_TRY(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+    return PP_OK;
+}
+
+int pp_malloc(void** dptr, size_t bytes) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!dptr) return fail(PP_ERR_INVALID, "null output pointer");
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 256;
+    HIP_TRY(hipMalloc(dptr, bytes), "hipMalloc");
+    return PP_OK;
+}
+
+int pp_free(void* dptr) {
+    if (!dptr) return PP_OK;
+    HIP_TRY(hipFree(dptr), "hipFree");
+    return PP_OK;
+}
+
+int pp_memcpy_h2d(void* dst, const void* src, size_t bytes, pp_stream s) {
+    if (!bytes) return PP_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_of(s)), "hipMemcpy H2D");
+    HIP_TRY(hipStreamSynchronize(stream_of(s)), "hipStreamSynchronize");
+    return PP_OK;
+}
+
+int pp_memcpy_d2h(void* dst, const void* src, size_t bytes, pp_stream s) {
+    if (!bytes) return PP_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_of(s)), "hipMemcpy D2H");
+    HIP_TRY(hipStreamSynchronize(stream_of(s)), "hipStreamSynchronize");
+    return PP_OK;
+}
+
+int pp_memcpy_d2d(void* dst, const void* src, size_t bytes, pp_stream s) {
+    if (!bytes) return PP_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_of(s)), "hipMemcpy D2D");
+    return PP_OK;
+}
+
+int pp_fill_f64(double* dptr, double value, int64_t count, pp_stream s) {
+    if (count <= 0) return PP_OK;
+    if (!dptr) return fail(PP_ERR_INVALID, "null pointer");
+    HIP_TRY(pp_strict::launch_fill(dptr, value, count, stream_of(s)), "fill kernel");
+    return PP_OK;
+}
+
+int pp_stream_create(pp_stream* s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipStream_t h;
+    HIP_TRY(hipStreamCreateWithFlags(&h, hipStreamNonBlocking), "hipStreamCreate");
+    *s = h;
+    return PP_OK;
+}
+
+int pp_stream_destroy(pp_stream s) {
+    if (s) HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(s)), "hipStreamDestroy");
+    return PP_OK;
+}
+
+int pp_stream_sync(pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(stream_of(s)), "hipStreamSynchronize");
+    return PP_OK;
+}
+
+int pp_device_sync(void) {
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return PP_OK;
+}
+
+int pp_event_create(pp_event* e) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipEvent_t h;
+    HIP_TRY(hipEventCreate(&h), "hipEventCreate");
+    *e = h;
+    return PP_OK;
+}
+
+int pp_event_destroy(pp_event e) {
+    if (e) HIP_TRY(hipEventDestroy(static_cast<hipEvent_t>(e)), "hipEventDestroy");
+    return PP_OK;
+}
+
+int pp_event_record(pp_event e, pp_stream s) {
+    HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(e), stream_of(s)), "hipEventRecord");
+    return PP_OK;
+}
+
+int pp_event_elapsed_ms(float* ms, pp_event start, pp_event stop) {
+    HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(stop)), "hipEventSynchronize");
+    HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)),
+            "hipEventElapsedTime");
+    return PP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Field maps
+// ---------------------------------------------------------------------------
+
+int pp_map_destroy(pp_map* m) {
+    if (!m) return PP_OK;
+    for (int k = 0; k < m->nalloc; ++k) hipFree(m->allocs[k]);
+    delete m;
+    return PP_OK;
+}
+
+static int map_alloc(pp_map* m, size_t bytes, void** out) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc (field map)");
+    m->allocs[m->nalloc++] = p;
+    m->bytes += bytes;
+    *out = p;
+    return PP_OK;
+}
+
+int pp_map_create(const double* values, int32_t ncomp, int32_t nx, int32_t ny, int32_t nz, const double* gx,
+                  const double* gy, const double* gz, double fill_value, int32_t flags, pp_map** out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!out) return fail(PP_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (!values || !gx || !gy || !gz) return fail(PP_ERR_INVALID, "null map input");
+    if (ncomp < 1 || ncomp > 6) return fail(PP_ERR_INVALID, "ncomp must be 1..6, got %d", ncomp);
+    const int32_t n[3] = {nx, ny, nz};
+    const double* g[3] = {gx, gy, gz};
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 1) return fail(PP_ERR_INVALID, "axis %d has %d nodes", a, n[a]);
+        for (int i = 1; i < n[a]; ++i)
+            if (!(g[a][i] > g[a][i - 1]) || !isfinite(g[a][i]))
+                return fail(PP_ERR_INVALID, "axis %d nodes must be finite and strictly increasing", a);
+    }
+    const bool periodic = (flags & PP_MAP_PERIODIC_Z) != 0;
+    if (periodic && nz < 2) return fail(PP_ERR_INVALID, "periodic z needs at least 2 z nodes");
+    if ((int64_t)nx * ny * nz > (int64_t)1 << 31) return fail(PP_ERR_INVALID, "map too large");
+
+    pp_map* m = new pp_map();
+    memset(m, 0, sizeof(*m));
+    m->device = g_device;
+    MapDesc& d = m->d;
+    d.ncomp = ncomp;
+    d.fill = fill_value;
+    d.periodic = periodic ? 1 : 0;
+    d.uniform = 1;
+    std::vector<double> gext[3];
+    for (int a = 0; a < 3; ++a) {
+        d.n[a] = n[a];
+        gext[a].assign(g[a], g[a] + n[a]);
+        if (a == 2 && periodic) {
+            // extra node z0 + L, L = nz * dz, dz = (z_last - z0)/(nz - 1), repeats node 0
+            const double dz = (g[2][nz - 1] - g[2][0]) / (double)(nz - 1);
+            d.L = (double)nz * dz;
+            gext[2].push_back(g[2][0] + d.L);
+        }
+        d.ng[a] = (int32_t)gext[a].size();
+        d.lo[a] = gext[a].front();
+        d.hi[a] = gext[a].back();
+        d.nc[a] = d.n[a] > 1 ? d.ng[a] - 1 : 1;
+        if (d.n[a] > 1) {
+            const double span = d.hi[a] - d.lo[a];
+            const double h = span / (double)(d.ng[a] - 1);
+            d.inv_d[a] = (double)(d.ng[a] - 1) / span;
+            for (int i = 0; i < d.ng[a]; ++i) {
+                const double ideal = d.lo[a] + h * i;
+                if (fabs(gext[a][i] - ideal) > 0.25 * h) d.uniform = 0;
+            }
+        }
+    }
+    if (periodic) d.inv_nz = 1.0 / (double)d.nc[2];
+
+    // raw table [comp][nx][ny][ngz] (periodic: node nz repeats node 0)
+    const int64_t ngz = d.ng[2];
+    const int64_t per_comp = (int64_t)nx * ny * ngz;
+    std::vector<double> raw((size_t)(per_comp * ncomp));
+    for (int c = 0; c < ncomp; ++c)
+        for (int64_t i = 0; i < nx; ++i)
+            for (int64_t j = 0; j < ny; ++j) {
+                const double* src = values + ((int64_t)c * nx * ny + i * ny + j) * nz;
+                double* dst = raw.data() + c * per_comp + (i * ny + j) * ngz;
+                memcpy(dst, src, sizeof(double) * nz);
+                if (periodic) dst[nz] = src[0];
+            }
+    void* p;
+    hipStream_t s = g_stream;
+    for (int a = 0; a < 3; ++a) {
+        if ((rc = map_alloc(m, sizeof(double) * gext[a].size(), &p))) { pp_map_destroy(m); return rc; }
+        d.g[a] = static_cast<const double*>(p);
+        if (hipMemcpyAsync(p, gext[a].data(), sizeof(double) * gext[a].size(), hipMemcpyHostToDevice, s) != hipSuccess) {
+            pp_map_destroy(m);
+            return fail(PP_ERR_HIP, "map upload failed");
+        }
+    }
+    if ((rc = map_alloc(m, sizeof(double) * raw.size(), &p))) { pp_map_destroy(m); return rc; }
+    d.raw = static_cast<const double*>(p);
+    if (hipMemcpyAsync(p, raw.data(), sizeof(double) * raw.size(), hipMemcpyHostToDevice, s) != hipSuccess) {
+        pp_map_destroy(m);
+        return fail(PP_ERR_HIP, "map upload failed");
+    }
+    const int64_t ncell = (int64_t)d.nc[0] * d.nc[1] * d.nc[2];
+    double* c64 = nullptr;
+    float* c32 = nullptr;
+    if (!(flags & PP_MAP_NO_COEF)) {
+        if ((rc = map_alloc(m, sizeof(double) * ncell * ncomp * 8, &p))) { pp_map_destroy(m); return rc; }
+        c64 = static_cast<double*>(p);
+        if (!(flags & PP_MAP_NO_FP32)) {
+            if ((rc = map_alloc(m, sizeof(float) * ncell * ncomp * 8, &p))) { pp_map_destroy(m); return rc; }
+            c32 = static_cast<float*>(p);
+        }
+        hipError_t e = pp_strict::launch_build_coef(d, c64, c32, s);
+        if (e != hipSuccess) { pp_map_destroy(m); return hip_fail(e, "coefficient build"); }
+    }
+    d.coef64 = c64;
+    d.coef32 = c32;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { pp_map_destroy(m); return hip_fail(e, "map build"); }
+    *out = m;
+    return PP_OK;
+}
+
+int pp_map_bytes(const pp_map* m, size_t* bytes) {
+    if (!m || !bytes) return fail(PP_ERR_INVALID, "null argument");
+    *bytes = m->bytes;
+    return PP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Field terms and pushes
+// ---------------------------------------------------------------------------
+
+static int make_field(const pp_field& f, int precision, FieldDev& out) {
+    memset(&out, 0, sizeof(out));
+    if (f.kind != PP_FIELD_ZERO && f.kind != PP_FIELD_CONST && f.kind != PP_FIELD_MAP && f.kind != PP_FIELD_ARRAY)
+        return fail(PP_ERR_INVALID, "unknown field kind %d", f.kind);
+    if (f.kind == PP_FIELD_ARRAY && !f.array) return fail(PP_ERR_INVALID, "array field without an array");
+    if (f.rf < PP_RF_NONE || f.rf > PP_RF_SIN) return fail(PP_ERR_INVALID, "unknown rf mode %d", f.rf);
+    out.kind = f.kind == PP_FIELD_ZERO ? PP_FIELD_CONST : f.kind;
+    out.comp0 = f.comp0;
+    out.has_scale2 = f.has_scale2;
+    out.has_offset = f.has_offset;
+    out.rf = f.rf;
+    out.scale1 = f.scale1;
+    out.scale2 = f.scale2;
+    out.offset = f.offset;
+    for (int c = 0; c < 3; ++c) out.value[c] = f.kind == PP_FIELD_ZERO ? 0.0 : f.value[c];
+    out.array = f.kind == PP_FIELD_ARRAY ? f.array : nullptr;
+    if (f.kind == PP_FIELD_MAP) {
+        if (!f.map) return fail(PP_ERR_INVALID, "map field without a map");
+        if (f.map->device != g_device) return fail(PP_ERR_INVALID, "map lives on device %d, current is %d",
+                                                   f.map->device, g_device);
+        if (f.comp0 < 0 || f.comp0 + 3 > f.map->d.ncomp)
+            return fail(PP_ERR_INVALID, "components %d..%d outside a %d-component map", f.comp0, f.comp0 + 2,
+                        f.map->d.ncomp);
+        out.map = f.map->d;
+        if (precision != PP_PREC_FP64) {
+            if (!out.map.uniform && (out.map.n[0] > 1 || out.map.n[1] > 1 || out.map.n[2] > 1))
+                return fail(PP_ERR_UNSUPPORTED, "precision %d needs a uniform grid; use PP_PREC_FP64 for "
+                            "non-uniform grids", precision);
+            if (precision == PP_PREC_FP64_FAST && !out.map.coef64)
+                return fail(PP_ERR_UNSUPPORTED, "map was built without coefficient tables");
+            if (precision == PP_PREC_FP32_MIXED && !out.map.coef32)
+                return fail(PP_ERR_UNSUPPORTED, "map was built without FP32 tables");
+        }
+    }
+    return PP_OK;
+}
+
+int pp_field_eval(const pp_field* field, int32_t precision, const double* pts, int64_t m, double rf_value,
+                  double* out, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!field || m < 0) return fail(PP_ERR_INVALID, "bad arguments");
+    if (m == 0) return PP_OK;
+    if (!pts || !out) return fail(PP_ERR_INVALID, "null device pointer");
+    if (precision < PP_PREC_FP64 || precision > PP_PREC_FP32_MIXED)
+        return fail(PP_ERR_INVALID, "unknown precision %d", precision);
+    FieldDev f;
+    if ((rc = make_field(*field, precision, f))) return rc;
+    if (f.kind == PP_FIELD_ARRAY) return fail(PP_ERR_INVALID, "array fields are not evaluated by pp_field_eval");
+    hipError_t e = precision == PP_PREC_FP64 ? pp_strict::launch_field_eval(f, pts, m, rf_value, out, stream_of(s))
+                                             : pp_fast::launch_field_eval(f, precision, pts, m, rf_value, out,
+                                                                          stream_of(s));
+    if (e != hipSuccess) return hip_fail(e, "field eval launch");
+    return PP_OK;
+}
+
+int pp_push(const pp_push_args* a, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!a) return fail(PP_ERR_INVALID, "null args");
+    if (a->algorithm < PP_ALG_LEAPFROG || a->algorithm > PP_ALG_VAY)
+        return fail(PP_ERR_INVALID, "unknown algorithm %d", a->algorithm);
+    if (a->precision < PP_PREC_FP64 || a->precision > PP_PREC_FP32_MIXED)
+        return fail(PP_ERR_INVALID, "unknown precision %d", a->precision);
+    if (a->n < 0 || a->nsteps < 0 || a->step0 < 0) return fail(PP_ERR_INVALID, "negative size");
+    if (a->n == 0 || a->nsteps == 0) return PP_OK;
+    if (a->n > ((int64_t)1 << 40)) return fail(PP_ERR_INVALID, "too many particles");
+    if (!a->x || !a->y || !a->z || !a->vx || !a->vy || !a->vz) return fail(PP_ERR_INVALID, "null state pointer");
+    if (!isfinite(a->dt) || !isfinite(a->q_over_m)) return fail(PP_ERR_INVALID, "dt and q_over_m must be finite");
+    const bool kick_alg = a->algorithm == PP_ALG_BORIS || a->algorithm == PP_ALG_BORIS_REL || a->algorithm == PP_ALG_VAY;
+    if (a->kick_only && (!kick_alg || a->nsteps != 1))
+        return fail(PP_ERR_INVALID, "kick_only needs a Boris/Vay-type algorithm and nsteps == 1");
+    if (a->dt_pp && a->algorithm != PP_ALG_RK4 && a->algorithm != PP_ALG_RK4_REL)
+        return fail(PP_ERR_INVALID, "per-particle dt is supported by the RK4 family only (pusher.py:815-816)");
+    if (a->r_hist || a->v_hist) {
+        if (!a->r_hist || !a->v_hist) return fail(PP_ERR_INVALID, "r_hist and v_hist go together");
+        if (a->rec_every < 1 || a->n_records < 1) return fail(PP_ERR_INVALID, "rec_every and n_records must be >= 1");
+    }
+    if (a->block_size < 0 || a->block_size > 1024 || (a->block_size % 64) != 0)
+        return fail(PP_ERR_INVALID, "block_size must be a multiple of 64 in [64, 1024] (or 0)");
+    PushParams p;
+    memset(&p, 0, sizeof(p));
+    if ((rc = make_field(a->efield, a->precision, p.E))) return rc;
+    if ((rc = make_field(a->bfield, a->precision, p.B))) return rc;
+    const bool rf = p.E.rf != PP_RF_NONE || p.B.rf != PP_RF_NONE;
+    if (rf) {
+        if (!a->rf_table) return fail(PP_ERR_INVALID, "time-dependent field without an rf table");
+        const int64_t last = a->step0 + (a->kick_only ? 0 : a->nsteps - 1);
+        if (a->step0 < a->rf_step0 || last >= a->rf_step0 + a->rf_nsteps)
+            return fail(PP_ERR_INVALID, "rf table rows [%lld, %lld) do not cover steps [%lld, %lld]",
+                        (long long)a->rf_step0, (long long)(a->rf_step0 + a->rf_nsteps), (long long)a->step0,
+                        (long long)last);
+    }
+    const bool has_array = a->efield.kind == PP_FIELD_ARRAY || a->bfield.kind == PP_FIELD_ARRAY;
+    if (has_array && (a->algorithm == PP_ALG_RK4 || a->algorithm == PP_ALG_RK4_REL || (a->nsteps != 1)))
+        return fail(PP_ERR_INVALID, "array fields need a single-gather algorithm and nsteps == 1");
+    p.fused = (a->efield.kind == PP_FIELD_MAP && a->bfield.kind == PP_FIELD_MAP && a->efield.map == a->bfield.map) ? 1 : 0;
+    p.n = a->n;
+    p.x = a->x; p.y = a->y; p.z = a->z;
+    p.vx = a->vx; p.vy = a->vy; p.vz = a->vz;
+    p.alive = a->alive;
+    p.dt_pp = a->dt_pp;
+    p.dt = a->dt;
+    p.qm = a->q_over_m;
+    p.nsteps = a->kick_only ? 1 : a->nsteps;
+    p.step0 = a->step0;
+    p.kick_only = a->kick_only;
+    p.rf_table = a->rf_table;
+    p.rf_step0 = a->rf_step0;
+    p.r_hist = a->r_hist;
+    p.v_hist = a->v_hist;
+    p.rec_every = a->r_hist ? a->rec_every : 1;
+    p.n_records = a->r_hist ? a->n_records : 0;
+    p.clight = kClight;
+    p.c2 = kClight * kClight;     // CLIGHT ** 2 (Python float pow == c*c for this value)
+    p.vclamp = 0.9999 * kClight;  // pusher.py:276
+    for (int k = 0; k < 3; ++k) {
+        p.yc[k] = kYoshidaC[k];
+        p.yd[k] = kYoshidaD[k];
+    }
+    hipError_t e = a->precision == PP_PREC_FP64
+                       ? pp_strict::launch_push(a->algorithm, p, a->block_size, stream_of(s))
+                       : pp_fast::launch_push(a->algorithm, a->precision, p, a->block_size, stream_of(s));
+    if (e != hipSuccess) return hip_fail(e, "push launch");
+    return PP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-step kernels and conversions
+// ---------------------------------------------------------------------------
+
+static int batch_launch(pp_strict::BatchArgs q, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (q.m < 0) return fail(PP_ERR_INVALID, "negative size");
+    if (q.m == 0) return PP_OK;
+    const bool need_b = q.which != PP_BK_CLAMP && q.which != PP_BK_SCALEOFF;
+    const bool need_c = q.which == PP_BK_BORIS || q.which == PP_BK_VAY || q.which == PP_BK_RK4REL ||
+                        q.which == PP_BK_LORENTZ || q.which == PP_BK_RK4COMB;
+    const bool need_de = q.which == PP_BK_RK4COMB;
+    if (!q.a || !q.out || (need_b && !q.b) || (need_c && !q.c) || (need_de && (!q.d || !q.e)))
+        return fail(PP_ERR_INVALID, "null device pointer");
+    q.c2 = kClight * kClight;
+    q.clight = kClight;
+    q.vclamp = 0.9999 * kClight;
+    hipError_t e = pp_strict::launch_batch_kernel(q, stream_of(s));
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+    return PP_OK;
+}
+
+static pp_strict::BatchArgs bargs(int which, const double* a, const double* b, const double* c, int64_t m,
+                                  double* out) {
+    pp_strict::BatchArgs q;
+    memset(&q, 0, sizeof(q));
+    q.which = which;
+    q.a = a;
+    q.b = b;
+    q.c = c;
+    q.m = m;
+    q.out = out;
+    return q;
+}
+
+int pp_boris_push_batch(const double* v, const double* E, const double* B, int64_t m, double dt, double q_over_m,
+                        double* v_out, pp_stream s) {
+    auto q = bargs(PP_BK_BORIS, v, E, B, m, v_out);
+    q.s1 = dt;
+    q.qm = q_over_m;
+    return batch_launch(q, s);
+}
+
+int pp_vay_push_batch(const double* v, const double* E, const double* B, int64_t m, double dt, double q_over_m,
+                      double* v_out, pp_stream s) {
+    auto q = bargs(PP_BK_VAY, v, E, B, m, v_out);
+    q.s1 = dt;
+    q.qm = q_over_m;
+    return batch_launch(q, s);
+}
+
+int pp_rk4_rel_dbetagamma_dt_batch(const double* v, const double* E, const double* B, int64_t m, double q_over_m,
+                                   double* out, pp_stream s) {
+    auto q = bargs(PP_BK_RK4REL, v, E, B, m, out);
+    q.qm = q_over_m;
+    return batch_launch(q, s);
+}
+
+int pp_position_update_batch(const double* r, const double* v, int64_t m, double dt, double* r_out, pp_stream s) {
+    auto q = bargs(PP_BK_POS, r, v, nullptr, m, r_out);
+    q.s1 = dt;
+    return batch_launch(q, s);
+}
+
+int pp_lorentz_batch(const double* v, const double* E, const double* B, int64_t m, double q_over_m, double* out,
+                     pp_stream s) {
+    auto q = bargs(PP_BK_LORENTZ, v, E, B, m, out);
+    q.qm = q_over_m;
+    return batch_launch(q, s);
+}
+
+int pp_axpy_batch(const double* a, double s_, const double* s_pp, const double* b, int64_t m, double* out,
+                  pp_stream s) {
+    auto q = bargs(PP_BK_AXPY, a, b, nullptr, m, out);
+    q.s1 = s_;
+    q.s_pp = s_pp;
+    return batch_launch(q, s);
+}
+
+int pp_rk4_combine_batch(const double* a, double w, const double* w_pp, const double* k1, const double* k2,
+                         const double* k3, const double* k4, int64_t m, double* out, pp_stream s) {
+    auto q = bargs(PP_BK_RK4COMB, a, k1, k2, m, out);
+    q.d = k3;
+    q.e = k4;
+    q.s1 = w;
+    q.s_pp = w_pp;
+    return batch_launch(q, s);
+}
+
+int pp_clamp_speed_batch(const double* v, int64_t m, double* out, pp_stream s) {
+    return batch_launch(bargs(PP_BK_CLAMP, v, nullptr, nullptr, m, out), s);
+}
+
+int pp_scale_offset_batch(const double* a, double scale, int32_t has_scale, double offset, int32_t has_offset,
+                          int64_t m, double* out, pp_stream s) {
+    auto q = bargs(PP_BK_SCALEOFF, a, nullptr, nullptr, m, out);
+    q.s1 = scale;
+    q.f1 = has_scale;
+    q.s2 = offset;
+    q.f2 = has_offset;
+    return batch_launch(q, s);
+}
+
+static int util_common(int which, double* a, double* b, double* c, double* aos, int64_t m, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (m < 0) return fail(PP_ERR_INVALID, "negative size");
+    if (m == 0) return PP_OK;
+    if (!a || !b || !c || ((which == PP_UT_AOS2SOA || which == PP_UT_SOA2AOS) && !aos))
+        return fail(PP_ERR_INVALID, "null device pointer");
+    hipError_t e = pp_strict::launch_util(which, a, b, c, aos, m, kClight * kClight, stream_of(s));
+    if (e != hipSuccess) return hip_fail(e, "conversion kernel launch");
+    return PP_OK;
+}
+
+int pp_aos_to_soa(const double* aos, int64_t m, double* a, double* b, double* c, pp_stream s) {
+    return util_common(PP_UT_AOS2SOA, a, b, c, const_cast<double*>(aos), m, s);
+}
+
+int pp_soa_to_aos(const double* a, const double* b, const double* c, int64_t m, double* aos, pp_stream s) {
+    return util_common(PP_UT_SOA2AOS, const_cast<double*>(a), const_cast<double*>(b), const_cast<double*>(c), aos, m, s);
+}
+
+int pp_v_to_u(double* vx, double* vy, double* vz, int64_t m, pp_stream s) {
+    return util_common(PP_UT_V2U, vx, vy, vz, nullptr, m, s);
+}
+
+int pp_u_to_v(double* ux, double* uy, double* uz, int64_t m, pp_stream s) {
+    return util_common(PP_UT_U2V, ux, uy, uz, nullptr, m, s);
+}
+
+}  // extern "C"

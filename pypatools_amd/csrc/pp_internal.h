@@ -1,0 +1,105 @@
+// Internal host/device structures of libpypat_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pypat_hip.h"
+
+// Device view of one field map.  Node arrays g[a] and the raw value table are
+// what the strict FP64 gather reads; coef64/coef32 are the per-cell trilinear
+// coefficient tables of the fast paths.
+struct MapDesc {
+    int32_t n[3];       // nodes per axis as given (1 = inactive axis)
+    int32_t ng[3];      // length of g[a] (z gets one extra node when periodic)
+    int32_t nc[3];      // cells per axis in the coefficient tables (1 on inactive axes)
+    int32_t ncomp;      // components stored (3 or 6)
+    int32_t periodic;   // z wraps with period L
+    int32_t uniform;    // every active axis is close to uniform: guess-and-correct lookup is exact
+    double lo[3];       // g[a][0]
+    double hi[3];       // g[a][ng-1]
+    double inv_d[3];    // (ng-1) / (hi-lo) on active axes, 0 otherwise
+    double L;           // z period (periodic maps)
+    double fill;        // value outside the grid
+    double inv_nz;      // 1 / nc[2] (periodic maps)
+    const double* g[3];
+    const double* raw;      // [comp][nx][ny][ng_z] float64
+    const double* coef64;   // [cell][comp][8] float64, cell = (i*nc1 + j)*nc2 + k
+    const float* coef32;    // same layout, float32
+};
+
+// Device form of one pp_field term.
+struct FieldDev {
+    int32_t kind;       // pp_field_kind
+    int32_t comp0;
+    int32_t has_scale2;
+    int32_t has_offset;
+    int32_t rf;         // pp_rf_mode
+    int32_t _pad;
+    double scale1, scale2, offset;
+    double value[3];
+    const double* array;    // PP_FIELD_ARRAY: (n, 3)
+    MapDesc map;
+};
+
+struct PushParams {
+    int64_t n;
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    uint8_t* alive;
+    const double* dt_pp;
+    double dt, qm;
+    int64_t nsteps, step0;
+    FieldDev E, B;
+    int32_t fused;          // E and B read the same map: one cell lookup for both
+    int32_t kick_only;
+    const double* rf_table;
+    int64_t rf_step0;
+    double* r_hist; double* v_hist;
+    int64_t rec_every, n_records;
+    // constants computed on the host with the reference's Python expressions
+    double clight, c2, vclamp;
+    double yc[3], yd[3];    // Yoshida coefficients, pusher.py:176-181
+};
+
+// Host-side map object behind the opaque pp_map handle.
+struct pp_map {
+    MapDesc d;
+    size_t bytes;
+    void* allocs[6];
+    int nalloc;
+    int device;
+};
+
+namespace pp_strict {
+hipError_t launch_push(int alg, const PushParams& p, int block, hipStream_t s);
+hipError_t launch_field_eval(const FieldDev& f, const double* pts, int64_t m, double rf, double* out,
+                             hipStream_t s);
+struct BatchArgs {
+    int which;
+    const double* a; const double* b; const double* c; const double* d; const double* e;
+    const double* s_pp;
+    int64_t m;
+    double s1, s2, qm;
+    int32_t f1, f2;
+    double c2, clight, vclamp;
+    double* out;
+};
+hipError_t launch_batch_kernel(const BatchArgs& args, hipStream_t s);
+hipError_t launch_util(int which, double* a, double* b, double* c, double* aos, int64_t m, double c2,
+                       hipStream_t s);
+hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
+hipError_t launch_build_coef(const MapDesc& d, double* c64, float* c32, hipStream_t s);
+}  // namespace pp_strict
+
+namespace pp_fast {
+hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStream_t s);
+hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int64_t m, double rf, double* out,
+                             hipStream_t s);
+}  // namespace pp_fast
+
+enum {
+    PP_BK_BORIS = 0, PP_BK_VAY = 1, PP_BK_RK4REL = 2, PP_BK_POS = 3, PP_BK_LORENTZ = 4, PP_BK_AXPY = 5,
+    PP_BK_RK4COMB = 6, PP_BK_CLAMP = 7, PP_BK_SCALEOFF = 8
+};
+enum { PP_UT_AOS2SOA = 0, PP_UT_SOA2AOS = 1, PP_UT_V2U = 2, PP_UT_U2V = 3 };

@@ -1,0 +1,93 @@
+"""Host-side API behaviour that needs no GPU: names, errors, species, constants."""
+import os
+
+import numpy as np
+import pytest
+
+import pypatools_amd as pa
+from pypatools_amd import constants as C
+from pypatools_amd.engine import build_rf_table
+from oracle.field import ORF, OField
+
+
+def test_constants_match_scipy():
+    import scipy.constants as sc
+    assert C.CLIGHT == sc.speed_of_light
+    assert C.ECHARGE == sc.elementary_charge
+    assert C.AMU_MEV == sc.value("atomic mass constant energy equivalent in MeV")
+    assert C.EMASS_MEV == sc.value("electron mass energy equivalent in MeV")
+    assert C.PMASS_MEV == sc.value("proton mass energy equivalent in MeV")
+    assert C.MUMASS_MEV == sc.value("muon mass energy equivalent in MeV")
+
+
+def test_species_match_reference_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "species_qm.npz"))
+    for i, name in enumerate(g["names"]):
+        sp = pa.IonSpecies(str(name))
+        assert sp.q_over_m == g["q_over_m"][i]
+        assert sp.mass_mev == g["mass_mev"][i]
+        assert sp.mass_kg == g["mass_kg"][i]
+    with pytest.raises(ValueError):
+        pa.IonSpecies("unobtainium")
+    custom = pa.IonSpecies("X", a=3.0, z=1.0, q=1.0)
+    assert custom.q_over_m == 1.0 * C.ECHARGE / custom.mass_kg
+
+
+class _Ion:
+    name = "H2_1+"
+    q_over_m = 4.79e7
+
+
+def test_pusher_names_errors_and_repr():
+    p = pa.Pusher(_Ion(), "BORIS")
+    assert p.algorithm == "boris" and not p.relativistic
+    assert pa.Pusher(_Ion(), "vay_rel").relativistic
+    assert pa.Pusher(_Ion(), "vay").relativistic
+    with pytest.raises(ValueError, match=r"Unknown algorithm 'euler'\. Must be one of \["):
+        pa.Pusher(_Ion(), "euler")
+    with pytest.raises(ValueError):
+        pa.Pusher(_Ion(), "boris", precision="fp16")
+    assert repr(p).startswith("Pusher(ion=H2_1+, algorithm='boris', relativistic=False")
+    for a in pa.Pusher.ALGORITHMS[:7]:
+        assert a in ["leapfrog", "boris", "rk4", "yoshida", "vay_rel", "rk4_rel", "yoshida_rel"]
+
+
+def test_change_algorithm_drops_electrode_assembly():
+    p = pa.Pusher(_Ion(), "boris", electrode_assembly=object())
+    p.change_algorithm("RK4")
+    assert p.algorithm == "rk4" and p.elec_assy is None
+    with pytest.raises(ValueError):
+        p.change_algorithm("nope")
+
+
+def test_leapfrog_redirect_for_energetic_ion():
+    class Hot(_Ion):
+        energy_mev = 0.5
+    with pytest.warns(UserWarning):
+        p = pa.Pusher(Hot(), "leapfrog")
+    assert p.algorithm == "vay_rel"
+
+
+def test_field_validation_without_device():
+    with pytest.raises(ValueError, match="Unknown unit"):
+        pa.Field(units="furlong")
+    with pytest.raises(ValueError, match="Unknown backend"):
+        pa.Field.from_arrays({"x": [0.0, 1.0]}, {"x": np.zeros(2)}, interpolator_backend="magic")
+    with pytest.raises(NotImplementedError):
+        pa.get_interpolator((np.arange(3.0),), np.zeros(3), method="cubic")
+    f = pa.Field.from_arrays({"x": np.arange(3.0), "y": [0.0], "z": np.arange(4.0)},
+                             {c: np.zeros((3, 4)) for c in "xyz"})
+    assert f.dim == 2
+    assert isinstance(2.0 * f, pa.ScaledField) and isinstance(f - f, pa.CompositeField)
+
+
+def test_rf_table_equals_oracle_factors():
+    rf = pa.RFField(pa.Field(dim=0, field={"x": 1.0, "y": 0.0, "z": 0.0}), 2 * np.pi * 1e8, 0.7, "cos")
+    orf = ORF(OField.constant(1.0, 0.0, 0.0), 2 * np.pi * 1e8, 0.7, "cos")
+    t0, dt = 1.5e-9, 2e-13
+    tab = build_rf_table(rf, t0, dt, 50)
+    for n in (0, 7, 49):
+        t_n = t0 + n * dt
+        for s, c in enumerate((0.0, 0.5, 0.5, 1.0)):
+            assert tab[n, s, 0] == orf.factor(t_n + c * dt)
+            assert tab[n, s, 1] == np.sin(orf.omega * (t_n + c * dt) + orf.phase)
