@@ -106,6 +106,8 @@ typedef struct pp_push_args {
     double* vx; double* vy; double* vz;
     uint8_t* alive;          /* optional; 0 = inactive: not pushed, not recorded */
     const double* dt_pp;     /* optional per-particle dt (RK4 family, pusher.py:815-816) */
+    const int32_t* ids;      /* optional: original particle index of each slot after pp_sort_state;
+                                history rows are written at ids[i] */
     double dt;               /* [s] */
     double q_over_m;         /* [C/kg] */
     int64_t nsteps;          /* steps in this call */
@@ -212,6 +214,17 @@ int pp_rk4_combine_batch(const double* a, double w, const double* w_pp, const do
 int pp_clamp_speed_batch(const double* v, int64_t m, double* out, pp_stream s);   /* pusher.py:886-890 */
 int pp_scale_offset_batch(const double* a, double scale, int32_t has_scale, double offset, int32_t has_offset,
                           int64_t m, double* out, pp_stream s);          /* scale*a + offset, field.py:690 */
+
+/* Cell sort (no reference counterpart: the reference's loops have no memory
+   locality to manage).  Reorders the SoA state of `a` (x..vz) by the map cell of
+   each particle -- efield's map, else bfield's -- and permutes `ids` alongside,
+   so a wavefront's gathers hit the same cache lines.  pp_unsort_state restores
+   the original order (ids -> 0..n-1).  Workspace: pp_sort_workspace_bytes(n).
+   No-op when neither field is a map. */
+int pp_sort_workspace_bytes(int64_t n, size_t* bytes);
+int pp_sort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
+int pp_unsort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
+int pp_iota_i32(int32_t* ids, int64_t n, pp_stream s);
 
 /* AoS (M,3) <-> SoA and v <-> u conversions (state staging for pp_push). */
 int pp_aos_to_soa(const double* aos, int64_t m, double* a, double* b, double* c, pp_stream s);

@@ -41,7 +41,7 @@ class PPPushArgs(ctypes.Structure):
         ("algorithm", c_int32), ("precision", c_int32), ("n", c_int64),
         ("x", c_void_p), ("y", c_void_p), ("z", c_void_p),
         ("vx", c_void_p), ("vy", c_void_p), ("vz", c_void_p),
-        ("alive", c_void_p), ("dt_pp", c_void_p),
+        ("alive", c_void_p), ("dt_pp", c_void_p), ("ids", c_void_p),
         ("dt", ctypes.c_double), ("q_over_m", ctypes.c_double),
         ("nsteps", c_int64), ("step0", c_int64),
         ("efield", PPField), ("bfield", PPField),
@@ -106,6 +106,10 @@ SIGNATURES = {
     "pp_clamp_speed_batch": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "pp_scale_offset_batch": (c_int32, [c_void_p, ctypes.c_double, c_int32, ctypes.c_double, c_int32, c_int64,
                                         c_void_p, c_void_p]),
+    "pp_sort_workspace_bytes": (c_int32, [c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "pp_sort_state": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "pp_unsort_state": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "pp_iota_i32": (c_int32, [c_void_p, c_int64, c_void_p]),
     "pp_aos_to_soa": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_soa_to_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "pp_v_to_u": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),

@@ -116,8 +116,16 @@ DEV f3 rk4_rel_deriv32(f3 v, f3 E, f3 B, float qm, float inv_c2) {
     return ig * force - (vdf * ig * inv_c2) * v;
 }
 
-template <int ALG>
-DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64_t gs, const PushParams& p) {
+struct GenericFE32 {
+    const PushParams& p;
+    DEV void operator()(int64_t i, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B) const {
+        eval_fields32(p, i, x, y, z, gs, stage, E, B);
+    }
+};
+
+template <int ALG, class FE>
+DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64_t gs, const FE& fe,
+                    const AlgConst& p) {
     const float inv_c2 = (float)(1.0 / p.c2);
     f3 E, B;
     if constexpr (ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL) {
@@ -128,19 +136,19 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
             else return lorentz32(tof(vv), ee, bb, qm);
         };
         const double hdt = 0.5 * dt;
-        eval_fields32(p, i, r.x, r.y, r.z, gs, 0, E, B);
+        fe(i, r.x, r.y, r.z, gs, 0, E, B);
         const f3 k1v = deriv(v, E, B);
         const d3 r2 = r + hdt * v;
         const d3 v2 = v + hdt * tod(k1v);
-        eval_fields32(p, i, r2.x, r2.y, r2.z, gs, 1, E, B);
+        fe(i, r2.x, r2.y, r2.z, gs, 1, E, B);
         const f3 k2v = deriv(v2, E, B);
         const d3 r3 = r + hdt * v2;
         const d3 v3 = v + hdt * tod(k2v);
-        eval_fields32(p, i, r3.x, r3.y, r3.z, gs, 2, E, B);
+        fe(i, r3.x, r3.y, r3.z, gs, 2, E, B);
         const f3 k3v = deriv(v3, E, B);
         const d3 r4 = r + dt * v3;
         const d3 v4 = v + dt * tod(k3v);
-        eval_fields32(p, i, r4.x, r4.y, r4.z, gs, 3, E, B);
+        fe(i, r4.x, r4.y, r4.z, gs, 3, E, B);
         const f3 k4v = deriv(v4, E, B);
         const double w = dt / 6.0;
         r = r + w * (((v + 2.0 * v2) + 2.0 * v3) + v4);
@@ -149,7 +157,7 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
         if constexpr (REL) v = clamp_speed(v, p.vclamp);
         vrec = v;
     } else {
-        eval_fields32(p, i, r.x, r.y, r.z, gs, 0, E, B);
+        fe(i, r.x, r.y, r.z, gs, 0, E, B);
         if constexpr (ALG == PP_ALG_BORIS) {
             const f3 dE = h * E;
             const d3 vm = addf(v, dE);
@@ -219,10 +227,10 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
     }
 }
 
-template <int ALG>
-DEV d3 kick_mixed(int64_t i, d3 r, d3 v, double dt, int64_t gs, const PushParams& p) {
+template <int ALG, class FE>
+DEV d3 kick_mixed(int64_t i, d3 r, d3 v, double dt, int64_t gs, const FE& fe, const AlgConst& p) {
     f3 E, B;
-    eval_fields32(p, i, r.x, r.y, r.z, gs, 0, E, B);
+    fe(i, r.x, r.y, r.z, gs, 0, E, B);
     const d3 Ed = tod(E), Bd = tod(B);
     if constexpr (ALG == PP_ALG_BORIS) return boris_kick(v, Ed, Bd, dt, p.qm);
     else if constexpr (ALG == PP_ALG_BORIS_REL) return boris_rel_kick(v, Ed, Bd, dt, p.qm, p.c2);
@@ -237,9 +245,11 @@ __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     d3 r = mk3(p.x[i], p.y[i], p.z[i]);
     d3 v = mk3(p.vx[i], p.vy[i], p.vz[i]);
     const double dt = p.dt_pp ? p.dt_pp[i] : p.dt;
+    const GenericFE32 fe{p};
+    const AlgConst ac = alg_const(p);
     if (p.kick_only) {
         if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
-            v = kick_mixed<ALG>(i, r, v, dt, p.step0, p);
+            v = kick_mixed<ALG>(i, r, v, dt, p.step0, fe, ac);
             p.vx[i] = v.x;
             p.vy[i] = v.y;
             p.vz[i] = v.z;
@@ -252,7 +262,7 @@ __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
-        step_mixed<ALG>(i, r, v, vrec, dt, h, p.step0 + s, p);
+        step_mixed<ALG>(i, r, v, vrec, dt, h, p.step0 + s, fe, ac);
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
             write_record(p, rec_idx, i, r, vrec);
@@ -280,6 +290,175 @@ __global__ void __launch_bounds__(256) k_field_eval32(const FieldDev f, const do
     out[3 * i + 2] = v.z;
 }
 
+// ---------------------------------------------------------------------------
+// Hot path: one fused E+B map, uniform grid, no RF/offset.  The map is read
+// through a buffer resource with 32-bit offsets (one base in SGPRs, loads past
+// the table return 0).
+// ---------------------------------------------------------------------------
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBufferDword3 = 0x00020000;   // gfx9 raw buffer descriptor word 3
+
+DEV float4 ld4f(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+
+DEV double2 ld2d(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    double2 d;
+    d.x = __hiloint2double((int)v.y, (int)v.x);
+    d.y = __hiloint2double((int)v.w, (int)v.z);
+    return d;
+}
+
+DEV float ceval(float4 lo, float4 hi, float tx, float ty, float tz) {
+    const float p1 = fmaf(ty, hi.x, lo.y);
+    const float p2 = fmaf(tx, hi.w, hi.z);
+    const float p3 = fmaf(tx, hi.y, lo.w);
+    const float p4 = fmaf(ty, p2, p3);
+    const float r = fmaf(ty, lo.z, fmaf(tx, p1, lo.x));
+    return fmaf(tz, p4, r);
+}
+
+DEV double cevald(double2 ab, double2 cd, double2 ef, double2 gh, double tx, double ty, double tz) {
+    const double p1 = fma(ty, ef.x, ab.y);
+    const double p2 = fma(tx, gh.y, gh.x);
+    const double p3 = fma(tx, ef.y, cd.y);
+    const double p4 = fma(ty, p2, p3);
+    const double r = fma(ty, cd.x, fma(tx, p1, ab.x));
+    return fma(tz, p4, r);
+}
+
+// cell index and weights on one non-periodic axis, u-space bounds test
+DEV uint32_t hot_axis(double x, double id, double c, int nc, double& t, bool& ok) {
+    const double u = fma(x, id, c);
+    ok = ok && (u >= 0.0) && (u <= (double)nc);
+    const double fu = fmin(fmax(floor(u), 0.0), (double)(nc - 1));
+    t = fmin(fmax(u - fu, 0.0), 1.0);
+    return (uint32_t)fu;
+}
+
+DEV uint32_t hot_axis_periodic(double z, double id, double c, int nc, double inv_n, double& t) {
+    const double u = fma(z, id, c);
+    const double fu = floor(u);
+    t = u - fu;
+    const double ncd = (double)nc;
+    double kd = fu - floor(fu * inv_n) * ncd;
+    if (kd >= ncd) kd -= ncd;
+    if (kd < 0.0) kd += ncd;
+    kd = fmin(fmax(kd, 0.0), ncd - 1.0);
+    return (uint32_t)kd;
+}
+
+template <bool PERIODIC>
+DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& tx, double& ty, double& tz,
+                        bool& ok) {
+    ok = true;
+    const uint32_t i = hot_axis(x, m.id0, m.c0, m.nc0, tx, ok);
+    const uint32_t j = hot_axis(y, m.id1, m.c1, m.nc1, ty, ok);
+    uint32_t k;
+    if constexpr (PERIODIC) k = hot_axis_periodic(z, m.id2, m.c2, m.nc2, m.inv_nz, tz);
+    else k = hot_axis(z, m.id2, m.c2, m.nc2, tz, ok);
+    return (i * (uint32_t)m.nc1 + j) * (uint32_t)m.nc2 + k;
+}
+
+template <bool PERIODIC>
+struct HotFE32 {
+    const HotMap& m;
+    __amdgpu_buffer_rsrc_t rs;
+    DEV void operator()(int64_t, double x, double y, double z, int64_t, int, f3& E, f3& B) const {
+        double tx, ty, tz;
+        bool ok;
+        const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
+        const uint32_t base = cell * (48u * 4u);
+        const float fx = (float)tx, fy = (float)ty, fz = (float)tz;
+        float v[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), fx, fy, fz);
+        const float fl = (float)m.fill, sE = (float)m.sE, sB = (float)m.sB;
+        E = mkf(sE * (ok ? v[0] : fl), sE * (ok ? v[1] : fl), sE * (ok ? v[2] : fl));
+        B = mkf(sB * (ok ? v[3] : fl), sB * (ok ? v[4] : fl), sB * (ok ? v[5] : fl));
+    }
+};
+
+template <bool PERIODIC>
+struct HotFE64 {
+    const HotMap& m;
+    __amdgpu_buffer_rsrc_t rs;
+    DEV void operator()(int64_t, double x, double y, double z, int64_t, int, d3& E, d3& B) const {
+        double tx, ty, tz;
+        bool ok;
+        const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
+        const uint32_t base = cell * (48u * 8u);
+        double v[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const uint32_t o = base + c * 64u;
+            v[c] = cevald(ld2d(rs, o), ld2d(rs, o + 16u), ld2d(rs, o + 32u), ld2d(rs, o + 48u), tx, ty, tz);
+        }
+        const double fl = m.fill;
+        E = mk3(m.sE * (ok ? v[0] : fl), m.sE * (ok ? v[1] : fl), m.sE * (ok ? v[2] : fl));
+        B = mk3(m.sB * (ok ? v[3] : fl), m.sB * (ok ? v[4] : fl), m.sB * (ok ? v[5] : fl));
+    }
+};
+
+DEV AlgConst hot_const(const HotParams& p) {
+    AlgConst a;
+    a.qm = p.qm;
+    a.c2 = p.c2;
+    a.clight = p.clight;
+    a.vclamp = p.vclamp;
+    for (int k = 0; k < 3; ++k) {
+        a.yc[k] = 0.0;
+        a.yd[k] = 0.0;
+    }
+    return a;
+}
+
+template <int ALG, bool MIXED, bool PERIODIC>
+__global__ void __launch_bounds__(256) k_hot(const HotParams p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p.m.coef), (short)0, (int)p.m.bytes, kBufferDword3);
+    const AlgConst ac = hot_const(p);
+    d3 r = mk3(p.x[i], p.y[i], p.z[i]);
+    d3 v = mk3(p.vx[i], p.vy[i], p.vz[i]);
+    const double dt = p.dt;
+    if (p.kick_only) {
+        if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
+            if constexpr (MIXED) v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC>{p.m, rs}, ac);
+            else v = kick64<ALG>(i, r, v, dt, p.step0, HotFE64<PERIODIC>{p.m, rs}, ac);
+            p.vx[i] = v.x;
+            p.vy[i] = v.y;
+            p.vz[i] = v.z;
+        }
+        return;
+    }
+    const float h = (float)(0.5 * p.qm * dt);
+    const bool rec_on = p.r_hist != nullptr;
+    int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
+    int64_t rec_idx = p.step0 / p.rec_every + 1;
+    d3 vrec = v;
+    for (int64_t s = 0; s < p.nsteps; ++s) {
+        if constexpr (MIXED) step_mixed<ALG>(i, r, v, vrec, dt, h, p.step0 + s, HotFE32<PERIODIC>{p.m, rs}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC>{p.m, rs}, ac);
+        if (rec_on && --rec_left == 0) {
+            rec_left = p.rec_every;
+            write_record_to(p.r_hist, p.v_hist, p.n, p.n_records, p.ids, rec_idx, i, r, vrec);
+            ++rec_idx;
+        }
+    }
+    p.x[i] = r.x;
+    p.y[i] = r.y;
+    p.z[i] = r.z;
+    p.vx[i] = v.x;
+    p.vy[i] = v.y;
+    p.vz[i] = v.z;
+}
+
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 #define PP_FAST_CASES(KERNEL)                                                                             \
@@ -303,6 +482,34 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
         PP_FAST_CASES(k_push64)
     } else {
         PP_FAST_CASES(k_push_mixed)
+    }
+    return hipGetLastError();
+}
+
+#define PP_HOT_CASE(A)                                                                                  \
+    case A:                                                                                             \
+        if (mixed) {                                                                                    \
+            if (periodic) hipLaunchKernelGGL((k_hot<A, true, true>), g, b, 0, s, p);                   \
+            else hipLaunchKernelGGL((k_hot<A, true, false>), g, b, 0, s, p);                           \
+        } else {                                                                                        \
+            if (periodic) hipLaunchKernelGGL((k_hot<A, false, true>), g, b, 0, s, p);                  \
+            else hipLaunchKernelGGL((k_hot<A, false, false>), g, b, 0, s, p);                          \
+        }                                                                                               \
+        break;
+
+hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int block, hipStream_t s) {
+    if (block <= 0) block = 256;
+    const dim3 g(grid_for(p.n, block)), b(block);
+    const bool mixed = prec == PP_PREC_FP32_MIXED;
+    switch (alg) {
+        PP_HOT_CASE(PP_ALG_LEAPFROG)
+        PP_HOT_CASE(PP_ALG_BORIS)
+        PP_HOT_CASE(PP_ALG_RK4)
+        PP_HOT_CASE(PP_ALG_VAY_REL)
+        PP_HOT_CASE(PP_ALG_RK4_REL)
+        PP_HOT_CASE(PP_ALG_BORIS_REL)
+        PP_HOT_CASE(PP_ALG_VAY)
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -85,7 +85,7 @@ int pp_init(int device) {
     if (device < 0 || device >= count) return fail(PP_ERR_INVALID, "device %d out of range (%d devices)", device, count);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     if (g_device != device) {
-        if (g_stream) hipStreamDestroy(g_stream);
+        if (g_stream) (void)hipStreamDestroy(g_stream);
         g_stream = nullptr;
         // default priority (no priority streams on this engine)
         HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -208,7 +208,7 @@ int pp_event_elapsed_ms(float* ms, pp_event start, pp_event stop) {
 
 int pp_map_destroy(pp_map* m) {
     if (!m) return PP_OK;
-    for (int k = 0; k < m->nalloc; ++k) hipFree(m->allocs[k]);
+    for (int k = 0; k < m->nalloc; ++k) (void)hipFree(m->allocs[k]);
     delete m;
     return PP_OK;
 }
@@ -373,6 +373,58 @@ static int make_field(const pp_field& f, int precision, FieldDev& out) {
     return PP_OK;
 }
 
+// The hot kernels cover the common case: E and B from one uniform 3-D map built
+// with E in components 0..2 and B in 3..5, no RF / offset, every particle active,
+// one dt, a single-gather algorithm or RK4, fast or mixed precision.
+static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& hp) {
+    if (a->precision == PP_PREC_FP64 || !p.fused || a->alive || a->dt_pp) return false;
+    if (a->algorithm == PP_ALG_YOSHIDA || a->algorithm == PP_ALG_YOSHIDA_REL) return false;
+    const FieldDev& E = p.E;
+    const FieldDev& B = p.B;
+    const MapDesc& m = E.map;
+    if (E.rf || B.rf || E.has_offset || B.has_offset || E.comp0 != 0 || B.comp0 != 3 || m.ncomp != 6) return false;
+    if (!m.uniform || m.n[0] < 2 || m.n[1] < 2 || m.n[2] < 2) return false;
+    const bool mixed = a->precision == PP_PREC_FP32_MIXED;
+    const void* coef = mixed ? (const void*)m.coef32 : (const void*)m.coef64;
+    if (!coef) return false;
+    const uint64_t bytes = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2] * 48 * (mixed ? 4 : 8);
+    if (bytes >= 0x7FFFFFFFull) return false;
+    memset(&hp, 0, sizeof(hp));
+    hp.x = p.x; hp.y = p.y; hp.z = p.z;
+    hp.vx = p.vx; hp.vy = p.vy; hp.vz = p.vz;
+    hp.ids = p.ids;
+    hp.n = p.n;
+    hp.nsteps = p.nsteps;
+    hp.step0 = p.step0;
+    hp.dt = p.dt;
+    hp.qm = p.qm;
+    hp.c2 = p.c2;
+    hp.clight = p.clight;
+    hp.vclamp = p.vclamp;
+    hp.r_hist = p.r_hist;
+    hp.v_hist = p.v_hist;
+    hp.rec_every = p.rec_every;
+    hp.n_records = p.n_records;
+    hp.kick_only = p.kick_only;
+    HotMap& h = hp.m;
+    h.coef = coef;
+    h.bytes = (uint32_t)bytes;
+    h.nc0 = m.nc[0];
+    h.nc1 = m.nc[1];
+    h.nc2 = m.nc[2];
+    h.id0 = m.inv_d[0];
+    h.id1 = m.inv_d[1];
+    h.id2 = m.inv_d[2];
+    h.c0 = -m.lo[0] * m.inv_d[0];
+    h.c1 = -m.lo[1] * m.inv_d[1];
+    h.c2 = -m.lo[2] * m.inv_d[2];
+    h.inv_nz = m.inv_nz;
+    h.sE = E.has_scale2 ? E.scale1 * E.scale2 : E.scale1;
+    h.sB = B.has_scale2 ? B.scale1 * B.scale2 : B.scale1;
+    h.fill = m.fill;
+    return true;
+}
+
 int pp_field_eval(const pp_field* field, int32_t precision, const double* pts, int64_t m, double rf_value,
                   double* out, pp_stream s) {
     int rc = ensure_init();
@@ -447,6 +499,7 @@ int pp_push(const pp_push_args* a, pp_stream s) {
     p.rf_step0 = a->rf_step0;
     p.r_hist = a->r_hist;
     p.v_hist = a->v_hist;
+    p.ids = a->ids;
     p.rec_every = a->r_hist ? a->rec_every : 1;
     p.n_records = a->r_hist ? a->n_records : 0;
     p.clight = kClight;
@@ -456,10 +509,69 @@ int pp_push(const pp_push_args* a, pp_stream s) {
         p.yc[k] = kYoshidaC[k];
         p.yd[k] = kYoshidaD[k];
     }
-    hipError_t e = a->precision == PP_PREC_FP64
-                       ? pp_strict::launch_push(a->algorithm, p, a->block_size, stream_of(s))
-                       : pp_fast::launch_push(a->algorithm, a->precision, p, a->block_size, stream_of(s));
+    HotParams hp;
+    hipError_t e;
+    if (hot_eligible(a, p, hp))
+        e = pp_fast::launch_hot(a->algorithm, a->precision, p.E.map.periodic != 0, hp, a->block_size, stream_of(s));
+    else if (a->precision == PP_PREC_FP64)
+        e = pp_strict::launch_push(a->algorithm, p, a->block_size, stream_of(s));
+    else
+        e = pp_fast::launch_push(a->algorithm, a->precision, p, a->block_size, stream_of(s));
     if (e != hipSuccess) return hip_fail(e, "push launch");
+    return PP_OK;
+}
+
+static const MapDesc* sort_map(const pp_push_args* a) {
+    if (a->efield.kind == PP_FIELD_MAP && a->efield.map) return &a->efield.map->d;
+    if (a->bfield.kind == PP_FIELD_MAP && a->bfield.map) return &a->bfield.map->d;
+    return nullptr;
+}
+
+int pp_sort_workspace_bytes(int64_t n, size_t* bytes) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n < 0 || !bytes) return fail(PP_ERR_INVALID, "bad arguments");
+    HIP_TRY(pp_sort::workspace_bytes(n, bytes), "sort workspace query");
+    return PP_OK;
+}
+
+static int sort_common(const pp_push_args* a, int32_t* ids, void* ws, size_t ws_bytes, pp_stream s, bool unsort) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!a || !ids) return fail(PP_ERR_INVALID, "null argument");
+    if (a->n < 0 || a->n > INT32_MAX) return fail(PP_ERR_INVALID, "sort supports 0 <= n < 2^31");
+    if (a->n == 0) return PP_OK;
+    if (!a->x || !a->y || !a->z || !a->vx || !a->vy || !a->vz) return fail(PP_ERR_INVALID, "null state pointer");
+    size_t need = 0;
+    HIP_TRY(pp_sort::workspace_bytes(a->n, &need), "sort workspace query");
+    if (!ws || ws_bytes < need) return fail(PP_ERR_INVALID, "sort workspace of %zu bytes needed", need);
+    double* arrs[6] = {a->x, a->y, a->z, a->vx, a->vy, a->vz};
+    if (unsort) {
+        HIP_TRY(pp_sort::unsort_state(arrs, ids, a->n, ws, ws_bytes, stream_of(s)), "unsort");
+        return PP_OK;
+    }
+    const MapDesc* m = sort_map(a);
+    if (!m) return PP_OK;
+    if ((uint64_t)m->nc[0] * m->nc[1] * m->nc[2] > 0xFFFFFFFFull) return fail(PP_ERR_UNSUPPORTED, "map too large");
+    HIP_TRY(pp_sort::sort_state(*m, arrs, ids, a->n, ws, ws_bytes, stream_of(s)), "sort");
+    return PP_OK;
+}
+
+int pp_sort_state(const pp_push_args* a, int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
+    return sort_common(a, ids, ws, ws_bytes, s, false);
+}
+
+int pp_unsort_state(const pp_push_args* a, int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
+    return sort_common(a, ids, ws, ws_bytes, s, true);
+}
+
+int pp_iota_i32(int32_t* ids, int64_t n, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n < 0 || n > INT32_MAX) return fail(PP_ERR_INVALID, "bad size");
+    if (n == 0) return PP_OK;
+    if (!ids) return fail(PP_ERR_INVALID, "null pointer");
+    HIP_TRY(pp_sort::iota(ids, n, stream_of(s)), "iota");
     return PP_OK;
 }
 

@@ -57,9 +57,36 @@ struct PushParams {
     int64_t rf_step0;
     double* r_hist; double* v_hist;
     int64_t rec_every, n_records;
+    const int32_t* ids;     // original particle index per slot (sorted state), or null
     // constants computed on the host with the reference's Python expressions
     double clight, c2, vclamp;
     double yc[3], yd[3];    // Yoshida coefficients, pusher.py:176-181
+};
+
+// Hot path: E and B from one uniform 3-D map (components 0..2 = E, 3..5 = B),
+// no RF / offset terms.  Compact, so the step loop keeps it in SGPRs.
+struct HotMap {
+    const void* coef;       // [cell][6][8] float32 or float64 coefficient table
+    uint32_t bytes;         // table size (buffer range: loads beyond it return 0)
+    int32_t nc0, nc1, nc2;  // cells per axis
+    double c0, c1, c2;      // -lo[a] * inv_d[a]
+    double id0, id1, id2;   // inv_d[a]
+    double inv_nz;          // 1 / nc2 (periodic z)
+    double sE, sB;          // scale1 * scale2 of E and B
+    double fill;            // value outside the grid (Field: 0)
+};
+
+struct HotParams {
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    const int32_t* ids;
+    int64_t n, nsteps, step0;
+    double dt, qm, c2, clight, vclamp;
+    double* r_hist; double* v_hist;
+    int64_t rec_every, n_records;
+    int32_t kick_only;
+    int32_t _pad;
+    HotMap m;
 };
 
 // Host-side map object behind the opaque pp_map handle.
@@ -94,9 +121,18 @@ hipError_t launch_build_coef(const MapDesc& d, double* c64, float* c32, hipStrea
 
 namespace pp_fast {
 hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStream_t s);
+hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int block, hipStream_t s);
 hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int64_t m, double rf, double* out,
                              hipStream_t s);
 }  // namespace pp_fast
+
+namespace pp_sort {
+hipError_t workspace_bytes(int64_t n, size_t* bytes);
+hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,
+                      hipStream_t s);
+hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes, hipStream_t s);
+hipError_t iota(int32_t* ids, int64_t n, hipStream_t s);
+}  // namespace pp_sort
 
 enum {
     PP_BK_BORIS = 0, PP_BK_VAY = 1, PP_BK_RK4REL = 2, PP_BK_POS = 3, PP_BK_LORENTZ = 4, PP_BK_AXPY = 5,

@@ -1,0 +1,140 @@
+// Cell sort of the particle state: particles that gather from the same map
+// cell end up in the same wavefront, so the 12 (FP32) or 24 (FP64) coefficient
+// loads of a wave touch a few cache lines instead of 64 (L1 hits instead of L2
+// traffic).  Keys are the coefficient-table cell of each particle; the radix
+// sort is hipCUB's; permutation and inverse permutation are plain gathers and
+// scatters.  `ids` carries each slot's original particle index so results and
+// history rows come back in the caller's order.
+#include <hipcub/hipcub.hpp>
+
+#include "pp_internal.h"
+
+namespace pp_sortk {
+
+__device__ __forceinline__ uint32_t key_axis(const MapDesc& m, int a, double x, bool periodic) {
+    if (m.n[a] <= 1) return 0;
+    const double u = (x - m.lo[a]) * m.inv_d[a];
+    const double nc = (double)m.nc[a];
+    double f = floor(u);
+    if (periodic) {
+        f = f - floor(f * m.inv_nz) * nc;
+        if (f >= nc) f -= nc;
+        if (f < 0.0) f += nc;
+    }
+    f = fmin(fmax(f, 0.0), nc - 1.0);   // outside / NaN -> a valid edge cell
+    return (uint32_t)f;
+}
+
+__global__ void __launch_bounds__(256) k_keys(const MapDesc m, const double* __restrict__ x,
+                                              const double* __restrict__ y, const double* __restrict__ z, int64_t n,
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ slots) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = key_axis(m, 0, x[i], false);
+    const uint32_t b = key_axis(m, 1, y[i], false);
+    const uint32_t c = key_axis(m, 2, z[i], m.periodic != 0);
+    keys[i] = (a * (uint32_t)m.nc[1] + b) * (uint32_t)m.nc[2] + c;
+    slots[i] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ perm,
+                                                    int64_t n, double* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ void __launch_bounds__(256) k_gather_i32(const int32_t* __restrict__ src, const uint32_t* __restrict__ perm,
+                                                    int64_t n, int32_t* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ void __launch_bounds__(256) k_scatter_f64(const double* __restrict__ src, const int32_t* __restrict__ ids,
+                                                     int64_t n, double* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (uint32_t)ids[i] < (uint64_t)n) dst[ids[i]] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_iota(int32_t* ids, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ids[i] = (int32_t)i;
+}
+
+}  // namespace pp_sortk
+
+namespace pp_sort {
+
+static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// workspace: keys_in, keys_out, slots_in, slots_out (4n each), tmp (8n), cub temp
+hipError_t workspace_bytes(int64_t n, size_t* bytes) {
+    size_t cub_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                      (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
+    if (e != hipSuccess) return e;
+    *bytes = 4 * align256(4 * (size_t)n) + align256(8 * (size_t)n) + align256(cub_bytes);
+    return hipSuccess;
+}
+
+static int key_bits(const MapDesc& m) {
+    const uint64_t ncell = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2];
+    int b = 1;
+    while (b < 32 && ((uint64_t)1 << b) < ncell) ++b;
+    return b;
+}
+
+hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,
+                      hipStream_t s) {
+    size_t need = 0;
+    hipError_t e = workspace_bytes(n, &need);
+    if (e != hipSuccess) return e;
+    if (ws_bytes < need) return hipErrorInvalidValue;
+    char* w = static_cast<char*>(ws);
+    const size_t q = align256(4 * (size_t)n);
+    uint32_t* keys_in = reinterpret_cast<uint32_t*>(w);
+    uint32_t* keys_out = reinterpret_cast<uint32_t*>(w + q);
+    uint32_t* slots_in = reinterpret_cast<uint32_t*>(w + 2 * q);
+    uint32_t* perm = reinterpret_cast<uint32_t*>(w + 3 * q);
+    double* tmp = reinterpret_cast<double*>(w + 4 * q);
+    char* cub_tmp = w + 4 * q + align256(8 * (size_t)n);
+    size_t cub_bytes = ws_bytes - (4 * q + align256(8 * (size_t)n));
+    hipLaunchKernelGGL(pp_sortk::k_keys, dim3(grid_for(n)), dim3(256), 0, s, m, arrs[0], arrs[1], arrs[2], n, keys_in,
+                       slots_in);
+    e = hipcub::DeviceRadixSort::SortPairs(cub_tmp, cub_bytes, keys_in, keys_out, slots_in, perm, (int)n, 0,
+                                           key_bits(m), s);
+    if (e != hipSuccess) return e;
+    for (int a = 0; a < 6; ++a) {
+        hipLaunchKernelGGL(pp_sortk::k_gather_f64, dim3(grid_for(n)), dim3(256), 0, s, arrs[a], perm, n, tmp);
+        e = hipMemcpyAsync(arrs[a], tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    int32_t* itmp = reinterpret_cast<int32_t*>(tmp);
+    hipLaunchKernelGGL(pp_sortk::k_gather_i32, dim3(grid_for(n)), dim3(256), 0, s, ids, perm, n, itmp);
+    e = hipMemcpyAsync(ids, itmp, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes, hipStream_t s) {
+    size_t need = 0;
+    hipError_t e = workspace_bytes(n, &need);
+    if (e != hipSuccess) return e;
+    if (ws_bytes < need) return hipErrorInvalidValue;
+    double* tmp = reinterpret_cast<double*>(static_cast<char*>(ws) + 4 * align256(4 * (size_t)n));
+    for (int a = 0; a < 6; ++a) {
+        hipLaunchKernelGGL(pp_sortk::k_scatter_f64, dim3(grid_for(n)), dim3(256), 0, s, arrs[a], ids, n, tmp);
+        e = hipMemcpyAsync(arrs[a], tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(pp_sortk::k_iota, dim3(grid_for(n)), dim3(256), 0, s, ids, n);
+    return hipGetLastError();
+}
+
+hipError_t iota(int32_t* ids, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(pp_sortk::k_iota, dim3(grid_for(n)), dim3(256), 0, s, ids, n);
+    return hipGetLastError();
+}
+
+}  // namespace pp_sort

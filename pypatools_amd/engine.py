@@ -14,6 +14,8 @@ NaN-filled history of nsteps // rec + 1 records, record 0 = initial state
 k*rec, the last velocity record overwritten with the resynchronised velocity,
 an ``active`` mask updated by the electrode test after each step.
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -29,6 +31,9 @@ RK4_FAMILY = ("rk4", "rk4_rel")
 
 # particle-steps per fused launch (keeps single launches around a second or less)
 DEFAULT_LAUNCH_WORK = 2 ** 30
+# steps between cell sorts of the state when a field is a gridded map
+DEFAULT_SORT_EVERY = 2000
+SORT_MIN_PARTICLES = 4096
 
 
 class ParticleState:
@@ -72,6 +77,53 @@ class ParticleState:
             _lib.call("pp_aos_to_soa", r_dev.ptr, self.n, self.x.ptr, self.y.ptr, self.z.ptr, None)
         if v_dev is not None:
             _lib.call("pp_aos_to_soa", v_dev.ptr, self.n, self.vx.ptr, self.vy.ptr, self.vz.ptr, None)
+
+    # -- cell sort -------------------------------------------------------------------
+    def ensure_ids(self):
+        if getattr(self, "ids", None) is None:
+            self.ids = DeviceArray((self.n,), np.int32)
+            _lib.call("pp_iota_i32", self.ids.ptr, self.n, None)
+            self.sorted = False
+        return self.ids
+
+    def _workspace(self):
+        ws = getattr(self, "ws", None)
+        if ws is None:
+            b = ctypes.c_size_t()
+            _lib.call("pp_sort_workspace_bytes", self.n, ctypes.byref(b))
+            ws = self.ws = DeviceArray((max(int(b.value), 8),), np.uint8)
+        return ws
+
+    def _state_args(self, fp=None):
+        a = _lib.PPPushArgs()
+        a.n = self.n
+        a.x, a.y, a.z = self.x.ptr, self.y.ptr, self.z.ptr
+        a.vx, a.vy, a.vz = self.vx.ptr, self.vy.ptr, self.vz.ptr
+        if fp is not None:
+            a.efield, a.bfield = fp.tE, fp.tB
+        return a
+
+    def sort(self, fp):
+        """Reorder by map cell (gather locality); a no-op when neither field is a map."""
+        if self.n == 0:
+            return
+        ids = self.ensure_ids()
+        ws = self._workspace()
+        _lib.call("pp_sort_state", self._state_args(fp), ids.ptr, ws.ptr, ws.nbytes, None)
+        self.sorted = True
+
+    def reset_order(self):
+        """Declare the current contents to be in original order (after an external reload)."""
+        if getattr(self, "ids", None) is not None:
+            _lib.call("pp_iota_i32", self.ids.ptr, self.n, None)
+        self.sorted = False
+
+    def unsort(self):
+        if self.n == 0 or not getattr(self, "sorted", False):
+            return
+        ws = self._workspace()
+        _lib.call("pp_unsort_state", self._state_args(), self.ids.ptr, ws.ptr, ws.nbytes, None)
+        self.sorted = False
 
     def v_to_u(self):
         _lib.call("pp_v_to_u", self.vx.ptr, self.vy.ptr, self.vz.ptr, self.n, None)
@@ -154,7 +206,8 @@ def _rf_source(f):
 class Tracker:
     """One track of n particles through (efield, bfield) on the device."""
 
-    def __init__(self, algorithm, precision, q_over_m, block_size=0, launch_work=DEFAULT_LAUNCH_WORK):
+    def __init__(self, algorithm, precision, q_over_m, block_size=0, launch_work=DEFAULT_LAUNCH_WORK,
+                 sort_every=DEFAULT_SORT_EVERY):
         if algorithm not in _lib.ALG_IDS:
             raise ValueError(f"unknown algorithm '{algorithm}'")
         if precision not in _lib.PRECISIONS:
@@ -164,6 +217,7 @@ class Tracker:
         self.qm = float(q_over_m)
         self.block_size = int(block_size)
         self.launch_work = int(launch_work)
+        self.sort_every = int(sort_every)   # steps between cell sorts (0: never) for map fields
 
     # -- kernel launch helpers ---------------------------------------------------
     def _args(self, st, fp, dt, nsteps, step0, tE=None, tB=None, rf=None, hist=None, kick=False, dt_pp=None):
@@ -174,6 +228,7 @@ class Tracker:
         a.x, a.y, a.z = st.x.ptr, st.y.ptr, st.z.ptr
         a.vx, a.vy, a.vz = st.vx.ptr, st.vy.ptr, st.vz.ptr
         a.alive = st.alive.ptr if st.alive is not None else None
+        a.ids = st.ids.ptr if getattr(st, "sorted", False) else None
         a.dt_pp = dt_pp.ptr if dt_pp is not None else None
         a.dt = float(dt)
         a.q_over_m = self.qm
@@ -242,14 +297,8 @@ class Tracker:
         st.positions_aos(_row(rh, 0, M))
         self._velocity_record(st, _row(vh, 0, M), ustate)
         if fp.direct and elec is None:
-            per_launch = max(1, self.launch_work // max(M, 1))
-            step = 0
-            while step < nsteps:
-                k = min(per_launch, nsteps - step)
-                if verbose:
-                    _progress(step, nsteps)
-                self._launch(self._args(st, fp, dt, k, step, rf=rf, hist=hist, dt_pp=dtp))
-                step += k
+            self._fused_loop(st, fp, nsteps, dt, 0, rf, hist, dtp, verbose)
+            st.unsort()
         else:
             self._slow_loop(st, fp, nsteps, dt, t0, rf, hist, elec, active, dtp, verbose)
         if alg in STAGGERED:
@@ -272,6 +321,38 @@ class Tracker:
             return r_hist, v_hist, active, st
         return r_hist, v_hist, active
 
+    def _uses_sort(self, st, fp, dtp):
+        maps = fp.tE.kind == _lib.FIELD_MAP or fp.tB.kind == _lib.FIELD_MAP
+        return self.sort_every > 0 and maps and st.n >= SORT_MIN_PARTICLES and dtp is None
+
+    def _fused_loop(self, st, fp, nsteps, dt, step0, rf, hist, dtp, verbose=False, steps_per_launch=None,
+                    timer=None):
+        """Chunks of fused steps; cell sort at the start and every sort_every steps."""
+        per = int(steps_per_launch or max(1, self.launch_work // max(st.n, 1)))
+        sort = self._uses_sort(st, fp, dtp)
+        if sort:
+            per = min(per, self.sort_every)
+        step, end = int(step0), int(step0) + int(nsteps)
+        since = None
+        while step < end:
+            if sort and (since is None or since >= self.sort_every):
+                st.sort(fp)
+                since = 0
+            k = min(per, end - step)
+            if sort:
+                k = min(k, self.sort_every - since)
+            if verbose:
+                _progress(step, end)
+            args = self._args(st, fp, dt, k, step, rf=rf, hist=hist, dt_pp=dtp)
+            if timer is not None:
+                timer(lambda: self._launch(args))
+            else:
+                self._launch(args)
+            step += k
+            if sort:
+                since += k
+        return step
+
     def run_resident(self, st, fp, nsteps, dt, step0=0, rf=None, steps_per_launch=None, timer=None):
         """Advance a device-resident state by nsteps (no history, no staggering).
 
@@ -280,18 +361,7 @@ class Tracker:
         """
         if not fp.direct:
             raise ValueError("run_resident needs fields the kernels read directly")
-        per = int(steps_per_launch or max(1, self.launch_work // max(st.n, 1)))
-        step = int(step0)
-        end = step + int(nsteps)
-        while step < end:
-            k = min(per, end - step)
-            args = self._args(st, fp, dt, k, step, rf=rf)
-            if timer is not None:
-                timer(lambda: self._launch(args))
-            else:
-                self._launch(args)
-            step += k
-        return step
+        return self._fused_loop(st, fp, nsteps, dt, step0, rf, None, None, False, steps_per_launch, timer)
 
     def single_step(self, r, v, efield, bfield, dt, t=0.0, dt_pp=None):
         """push_batch (pusher.py:699-801): one step, no staggering, no history; returns host (r, v)."""

@@ -26,7 +26,7 @@ class Pusher:
     ALGORITHMS = REFERENCE_ALGORITHMS + EXTENSION_ALGORITHMS
 
     def __init__(self, ion, algorithm="boris", use_numba=True, electrode_assembly=None, precision="fp64",
-                 block_size=0):
+                 block_size=0, sort_every=None):
         self.ion = ion
         self.q_over_m = ion.q_over_m
         algorithm = algorithm.lower()
@@ -46,9 +46,11 @@ class Pusher:
         self.elec_assy = electrode_assembly
         self.precision = precision
         self.block_size = block_size
+        self.sort_every = sort_every
 
     def _tracker(self):
-        return Tracker(self.algorithm, self.precision, self.q_over_m, self.block_size)
+        kw = {} if self.sort_every is None else {"sort_every": self.sort_every}
+        return Tracker(self.algorithm, self.precision, self.q_over_m, self.block_size, **kw)
 
     # ---- single particle ------------------------------------------------------------
     def push(self, r, v, efield, bfield, dt, t=0.0):
@@ -92,7 +94,7 @@ class Pusher:
         if algorithm not in self.ALGORITHMS:
             raise ValueError(f"Unknown algorithm '{algorithm}'. Must be one of {self.ALGORITHMS}")
         self.__init__(self.ion, algorithm=algorithm, use_numba=self.use_numba, precision=self.precision,
-                      block_size=self.block_size)
+                      block_size=self.block_size, sort_every=self.sort_every)
 
     def __repr__(self):
         return (f"Pusher(ion={self.ion.name}, algorithm='{self.algorithm}', "

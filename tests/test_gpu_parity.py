@@ -473,3 +473,39 @@ def test_errors_are_raised_not_faulted():
                                                                            "z": np.zeros((3, 2, 2))})(np.zeros((1, 3)))
     empty = pu.track_batch(np.zeros((0, 3)), np.zeros((0, 3)), z, z, 10, 1e-11, 5)
     assert empty[0].shape == (3, 0, 3)
+
+
+# --------------------------------------------------------------------------- cell sort
+
+
+@pytest.mark.parametrize("alg", ["boris", "rk4", "vay"])
+def test_strict_sorted_state_history_bitwise(alg):
+    """M >= 4096 turns the cell sort on: history rows go to the original particle index, results are unchanged."""
+    rng = np.random.default_rng(26)
+    grid, E, B = map_c2(16, 1e-3, 0.05, rng)
+    M = 6000
+    r0 = np.column_stack([rng.uniform(-6e-3, 6e-3, M), rng.uniform(-6e-3, 6e-3, M), rng.uniform(1e-3, 14e-3, M)])
+    v0 = rng.standard_normal((M, 3)) * 3e5
+    gp = pa.Pusher(_Ion(QM_P), alg, sort_every=40)
+    rh, vh, _ = gp.track_batch(r0, v0, pa.Field.from_arrays(grid, E), pa.Field.from_arrays(grid, B), 130, 1e-10, 25)
+    orh, ovh, _ = OraclePusher(QM_P, alg).track_batch(r0, v0, OField.from_arrays(grid, E), OField.from_arrays(grid, B),
+                                                       130, 1e-10, 25)
+    bitwise(rh, orh)
+    bitwise(vh, ovh)
+
+
+@pytest.mark.parametrize("prec", ["fp32_mixed", "fp64_fast"])
+@pytest.mark.parametrize("alg", ["boris", "vay", "rk4_rel"])
+def test_hot_kernel_sorted_equals_unsorted(prec, alg):
+    """Particles are independent: the sorted and the unsorted runs of the hot kernel agree bit for bit."""
+    rng = np.random.default_rng(27)
+    grid, E, B = map_c2(32, 1e-3, 0.0, rng)
+    M = 20000
+    r0 = np.column_stack([rng.normal(0, 2e-3, M), rng.normal(0, 2e-3, M), rng.normal(16e-3, 2e-3, M)])
+    v0 = np.column_stack([np.zeros(M), np.zeros(M), 2.19e7 * (1 + 1e-3 * rng.standard_normal(M))])
+    fe = pa.Field.from_arrays(grid, E, periodic_z=True)
+    fb = pa.Field.from_arrays(grid, B, periodic_z=True)
+    a = pa.Pusher(_Ion(QM_P), alg, precision=prec, sort_every=100).track_batch(r0, v0, fe, fb, 300, 5e-11, 60)
+    b = pa.Pusher(_Ion(QM_P), alg, precision=prec, sort_every=0).track_batch(r0, v0, fe, fb, 300, 5e-11, 60)
+    bitwise(a[0], b[0])
+    bitwise(a[1], b[1])
