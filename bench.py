@@ -44,6 +44,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--sort-every", type=int, default=2000, help="steps between cell sorts (0: never)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"])
+    ap.add_argument("--bunches", type=int, default=512)
+    ap.add_argument("--c4-scale", type=float, default=1.0, help="scale C4 bunch sizes (1.0 = the C4 config)")
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay the C4 window loop from a HIP graph")
+    ap.add_argument("--jobs", type=int, default=300)
+    ap.add_argument("--load", type=float, default=0.7, help="C5 offered load as a fraction of --capacity")
+    ap.add_argument("--capacity", type=float, default=1.0e11, help="C5 capacity (particle-steps/s) for lambda")
+    ap.add_argument("--pool", type=int, default=60_000_000, help="C5 particle pool (HBM budget)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the AoS->SoA copy kernel over this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -83,6 +92,100 @@ def cpu_baseline(w, seconds=15.0):
                       f"(vectorised over particles, single thread), {el:.1f} s"}
 
 
+def bench_c4(args):
+    """C4: 512 bunches (~4e7 protons/H2+/electrons), one batched launch per window, autotuned."""
+    import pypatools_amd as pa
+    from pypatools_amd.autotune import autotune_batch
+    from pypatools_amd.batch import BatchedTracker, c4_bunches
+    from pypatools_amd.device import synchronize
+    from pypatools_amd.workloads import c2_map
+    grid, E, B = c2_map(128)
+    fields = [(pa.Field.from_arrays(grid, E, periodic_z=True), pa.Field.from_arrays(grid, B, periodic_z=True))]
+    t = time.perf_counter()
+    bunches = c4_bunches(nbunch=args.bunches, scale=args.c4_scale)
+    gen_s = time.perf_counter() - t
+    tune = {}
+    if args.autotune:
+        sub = bunches[::8]
+        t = time.perf_counter()
+        cfg = autotune_batch(fields, sub, args.precision, sample_steps=600, use_cache=False)
+        tune = {"autotune_s": time.perf_counter() - t, "config": cfg}
+    else:
+        cfg = {"block_size": 256, "steps_per_launch": 500, "sort_every": 500}
+    bt = BatchedTracker(fields, args.precision, use_graph=args.graph, **cfg)
+    rates = []
+    for rep in range(args.warmup + args.steps):
+        bt.load(bunches)
+        synchronize()
+        t = time.perf_counter()
+        bt.run()
+        el = time.perf_counter() - t
+        if rep >= args.warmup:
+            rates.append(bt.particle_steps / el)
+    res = bt.results()
+    r_all = np.concatenate([r for r, _ in res])
+    inside = float(np.mean((np.abs(r_all[:, 0]) <= 0.0635) & (np.abs(r_all[:, 1]) <= 0.0635)))
+    value = float(np.median(rates))
+    out = {"metric": "particle-steps/sec on 1 MI355X (C4 batched persistent launch)", "value": value,
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * bt.particle_steps / value, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64" if args.precision == "fp32_mixed" else "f64",
+           "data": "synthetic",
+           "config": {"workload": f"C4: {len(bunches)} bunches, {sum(b.n for b in bunches)} particles, "
+                                  f"{bt.particle_steps:.3e} particle-steps, shared 128^3 map, per-bunch scale "
+                                  "0.5-1.5, species proton/H2+/electron, Boris or Vay, 1e3-1e4 steps",
+                      "precision": args.precision, "hip_graph": bool(args.graph), **cfg},
+           "extra": {"generate_s": gen_s, "in_map_fraction_end": inside, "rates": rates, **tune}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_c5(args):
+    """C5: Poisson arrivals, continuous batching, p50/p99 latency at a load set from measured capacity."""
+    import pypatools_amd as pa
+    from pypatools_amd.serving import TrackingServer, c5_jobs
+    from pypatools_amd.workloads import c2_map
+    grid, E, B = c2_map(128)
+    fields = []
+    for k in range(4):
+        s = 1.0 + 0.1 * k
+        Ek = {c: s * E[c] for c in "xyz"}
+        Bk = {c: (2.0 - s) * B[c] for c in "xyz"}
+        fields.append((pa.Field.from_arrays(grid, Ek, periodic_z=True), pa.Field.from_arrays(grid, Bk, periodic_z=True)))
+    srv = TrackingServer(fields, args.precision, chunk_steps=256, pool_particles=args.pool)
+    mean_work = 2.63e5 * 3.76e3       # E[N] E[steps] of the C5 distributions
+    capacity = args.capacity
+    lam = args.load * capacity / mean_work
+    jobs = c5_jobs(args.jobs, lam)
+    t0 = time.perf_counter()
+    for j in jobs:
+        j.arrival += t0
+    pending = list(jobs)
+    while pending or not srv.idle:
+        now = time.perf_counter()
+        while pending and pending[0].arrival <= now:
+            srv.submit(pending.pop(0))
+        if srv.idle and pending:
+            time.sleep(max(0.0, min(pending[0].arrival - time.perf_counter(), 0.05)))
+            continue
+        srv.round(now)
+    span = time.perf_counter() - t0
+    lat = np.array([j.latency for j in jobs])
+    work = sum(j.n * j.nsteps for j in jobs)
+    out = {"metric": "p99 job latency, continuous-batching tracking server on 1 MI355X (C5)",
+           "value": float(np.percentile(lat, 99)), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": span * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64" if args.precision == "fp32_mixed" else "f64", "data": "synthetic",
+           "config": {"workload": f"C5: {len(jobs)} jobs, Poisson arrivals at {lam:.2f} jobs/s "
+                                  f"({args.load:.0%} of {capacity:.2e} particle-steps/s capacity), N log-uniform "
+                                  "1e3-2e6, steps log-uniform 1e2-2e4, Boris/Vay/RK4, 4 resident 128^3 maps, "
+                                  "256-step chunks", "precision": args.precision, "pool_particles": srv.pool_n},
+           "extra": {"p50_s": float(np.percentile(lat, 50)), "p90_s": float(np.percentile(lat, 90)),
+                     "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+                     "throughput_particle_steps_per_s": work / span, "makespan_s": span, "rounds": srv.rounds,
+                     "arrival_rate_jobs_per_s": lam}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -92,6 +195,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine shards by particle "
               "index across GPUs by design but this build measures one GPU's share", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     import pypatools_amd as pa
     from pypatools_amd.device import DeviceArray, Event, synchronize

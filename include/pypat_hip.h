@@ -140,6 +140,8 @@ int pp_abi_version(void);
 int pp_get_constants(double* out, int32_t n);
 /* sizeof(pp_field), sizeof(pp_push_args) as compiled (binding layout check). */
 int pp_struct_sizes(int64_t* field_size, int64_t* push_args_size);
+/* sizeof(pp_bunch), sizeof(pp_batch_args). */
+int pp_batch_struct_sizes(int64_t* bunch_size, int64_t* batch_args_size);
 int pp_device_count(int* count);
 int pp_mem_info(size_t* free_bytes, size_t* total_bytes);  /* hipMemGetInfo */
 int pp_malloc(void** dptr, size_t bytes);
@@ -215,6 +217,58 @@ int pp_clamp_speed_batch(const double* v, int64_t m, double* out, pp_stream s); 
 int pp_scale_offset_batch(const double* a, double scale, int32_t has_scale, double offset, int32_t has_offset,
                           int64_t m, double* out, pp_stream s);          /* scale*a + offset, field.py:690 */
 
+/* ---- batched push of many independent bunches (extension: the reference
+        tracks one particle array per call) ------------------------------------ */
+/* One bunch: particles [offset, offset+count) of the shared SoA state. */
+typedef struct pp_bunch {
+    int64_t offset;
+    int64_t count;
+    int32_t algorithm;       /* BORIS, VAY, VAY_REL, BORIS_REL, RK4, RK4_REL or LEAPFROG */
+    int32_t nsteps;          /* total steps of this bunch */
+    int32_t map_index;       /* which of the launch's maps (0..nmaps-1) */
+    int32_t _pad;
+    double dt;
+    double q_over_m;
+    double escale;           /* multiplies the map's E components (0..2) */
+    double bscale;           /* multiplies the map's B components (3..5) */
+} pp_bunch;
+
+#define PP_MAX_BATCH_MAPS 4
+
+typedef struct pp_batch_args {
+    int32_t precision;       /* PP_PREC_FP64_FAST or PP_PREC_FP32_MIXED */
+    int32_t block_size;      /* = tile size used by pp_batch_tiles */
+    int64_t n;               /* particles in the state */
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;  /* v, or u = gamma v for BORIS_REL / VAY bunches */
+    const pp_bunch* bunches; /* device array */
+    int32_t nbunch;
+    int32_t nmaps;
+    const int32_t* tile_bunch;   /* device, from pp_batch_tiles */
+    const int64_t* tile_start;   /* device */
+    int64_t ntiles;
+    int64_t step0;           /* steps [step0, step0+nsteps) of every bunch (a bunch stops at its nsteps) */
+    int64_t nsteps;
+    const int64_t* step0_dev;  /* optional device scalar read instead of step0 (graph replays) */
+    const pp_map* maps[PP_MAX_BATCH_MAPS];   /* 6-component (E then B) uniform 3-D maps */
+} pp_batch_args;
+
+/* Host helper: split the bunches into tiles of `tile` particles that never
+   straddle two bunches.  tile_bunch/tile_start: host arrays with room for
+   sum(ceil(count/tile)) entries (query with NULL outputs). */
+int pp_batch_tiles(const pp_bunch* bunches, int32_t nbunch, int32_t tile, int32_t* tile_bunch,
+                   int64_t* tile_start, int64_t* ntiles);
+/* One launch advancing every bunch over [step0, step0+nsteps) (fused, state in registers). */
+int pp_push_batched(const pp_batch_args* a, pp_stream s);
+
+/* ---- HIP graphs: capture a stream's launches once, replay them ---------- */
+typedef void* pp_graph;
+int pp_graph_begin(pp_stream s);                    /* stream capture (the stream must not be NULL) */
+int pp_graph_end(pp_stream s, pp_graph* g);         /* end capture, instantiate */
+int pp_graph_launch(pp_graph g, pp_stream s);
+int pp_graph_destroy(pp_graph g);
+int pp_add_i64(int64_t* dptr, int64_t value, pp_stream s);   /* *dptr += value on the device (kernel) */
+
 /* Cell sort (no reference counterpart: the reference's loops have no memory
    locality to manage).  Reorders the SoA state of `a` (x..vz) by the map cell of
    each particle -- efield's map, else bfield's -- and permutes `ids` alongside,
@@ -223,6 +277,10 @@ int pp_scale_offset_batch(const double* a, double scale, int32_t has_scale, doub
    No-op when neither field is a map. */
 int pp_sort_workspace_bytes(int64_t n, size_t* bytes);
 int pp_sort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
+/* Same, for a batched state: the key is (group[i] << group_shift) | cell, so each
+   bunch keeps its slot range; the map is `map`. */
+int pp_sort_state_grouped(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
+                          int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
 int pp_unsort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
 int pp_iota_i32(int32_t* ids, int64_t n, pp_stream s);
 

@@ -51,6 +51,28 @@ class PPPushArgs(ctypes.Structure):
     ]
 
 
+class PPBunch(ctypes.Structure):
+    _fields_ = [
+        ("offset", c_int64), ("count", c_int64), ("algorithm", c_int32), ("nsteps", c_int32),
+        ("map_index", c_int32), ("_pad", c_int32), ("dt", ctypes.c_double), ("q_over_m", ctypes.c_double),
+        ("escale", ctypes.c_double), ("bscale", ctypes.c_double),
+    ]
+
+
+MAX_BATCH_MAPS = 4
+
+
+class PPBatchArgs(ctypes.Structure):
+    _fields_ = [
+        ("precision", c_int32), ("block_size", c_int32), ("n", c_int64),
+        ("x", c_void_p), ("y", c_void_p), ("z", c_void_p), ("vx", c_void_p), ("vy", c_void_p), ("vz", c_void_p),
+        ("bunches", c_void_p), ("nbunch", c_int32), ("nmaps", c_int32),
+        ("tile_bunch", c_void_p), ("tile_start", c_void_p), ("ntiles", c_int64),
+        ("step0", c_int64), ("nsteps", c_int64), ("step0_dev", c_void_p),
+        ("maps", c_void_p * MAX_BATCH_MAPS),
+    ]
+
+
 class HipError(RuntimeError):
     """A HIP runtime failure inside libpypat_hip.so."""
 
@@ -69,6 +91,7 @@ SIGNATURES = {
     "pp_abi_version": (c_int32, []),
     "pp_get_constants": (c_int32, [c_double_p, c_int32]),
     "pp_struct_sizes": (c_int32, [ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "pp_batch_struct_sizes": (c_int32, [ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "pp_device_count": (c_int32, [ctypes.POINTER(ctypes.c_int)]),
     "pp_mem_info": (c_int32, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "pp_malloc": (c_int32, [ctypes.POINTER(c_void_p), ctypes.c_size_t]),
@@ -110,6 +133,15 @@ SIGNATURES = {
     "pp_sort_state": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_unsort_state": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_iota_i32": (c_int32, [c_void_p, c_int64, c_void_p]),
+    "pp_batch_tiles": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, ctypes.POINTER(c_int64)]),
+    "pp_push_batched": (c_int32, [c_void_p, c_void_p]),
+    "pp_sort_state_grouped": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
+                                        ctypes.c_size_t, c_void_p]),
+    "pp_graph_begin": (c_int32, [c_void_p]),
+    "pp_graph_end": (c_int32, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "pp_graph_launch": (c_int32, [c_void_p, c_void_p]),
+    "pp_graph_destroy": (c_int32, [c_void_p]),
+    "pp_add_i64": (c_int32, [c_void_p, c_int64, c_void_p]),
     "pp_aos_to_soa": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_soa_to_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "pp_v_to_u": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
@@ -158,7 +190,10 @@ def layout_ok():
     """True if the ctypes structures match the compiled C layout."""
     a, b = c_int64(), c_int64()
     check(lib().pp_struct_sizes(ctypes.byref(a), ctypes.byref(b)))
-    return a.value == ctypes.sizeof(PPField) and b.value == ctypes.sizeof(PPPushArgs)
+    c, d = c_int64(), c_int64()
+    check(lib().pp_batch_struct_sizes(ctypes.byref(c), ctypes.byref(d)))
+    return (a.value == ctypes.sizeof(PPField) and b.value == ctypes.sizeof(PPPushArgs)
+            and c.value == ctypes.sizeof(PPBunch) and d.value == ctypes.sizeof(PPBatchArgs))
 
 
 def constants():

@@ -459,7 +459,93 @@ __global__ void __launch_bounds__(256) k_hot(const HotParams p) {
     p.vz[i] = v.z;
 }
 
+// ---------------------------------------------------------------------------
+// Batched push: one block = one tile of one bunch; the bunch's algorithm,
+// q/m, dt, step count, field scales and map come from a device table (uniform
+// per block, so the branches below are scalar).
+// ---------------------------------------------------------------------------
+
+template <int ALG, bool MIXED, bool PERIODIC>
+DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double dt, const HotMap& m,
+                     __amdgpu_buffer_rsrc_t rs, const AlgConst& ac) {
+    d3 vrec = v;
+    const float h = (float)(0.5 * ac.qm * dt);
+    for (int64_t s = 0; s < ns; ++s) {
+        if constexpr (MIXED) step_mixed<ALG>(i, r, v, vrec, dt, h, step0 + s, HotFE32<PERIODIC>{m, rs}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs}, ac);
+    }
+}
+
+template <bool MIXED, bool PERIODIC>
+__global__ void __launch_bounds__(256) k_batched(const BatchParams p) {
+    const int64_t t = blockIdx.x;
+    if (t >= p.ntiles) return;
+    const int32_t b = p.tile_bunch[t];
+    if (b < 0 || b >= p.nbunch) return;
+    const pp_bunch bd = p.bunches[b];
+    if (bd.map_index < 0 || bd.map_index >= p.nmaps) return;
+    const int64_t i = p.tile_start[t] + threadIdx.x;
+    if (i < bd.offset || i >= bd.offset + bd.count || i >= p.n) return;
+    const int64_t step0 = p.step0_dev ? *p.step0_dev : p.step0;
+    const int64_t end = bd.nsteps < step0 + p.nsteps ? (int64_t)bd.nsteps : step0 + p.nsteps;
+    if (end <= step0) return;
+    const int64_t ns = end - step0;
+    HotMap m = p.maps[bd.map_index];
+    m.sE *= bd.escale;
+    m.sB *= bd.bscale;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(m.coef), (short)0, (int)m.bytes, kBufferDword3);
+    AlgConst ac;
+    ac.qm = bd.q_over_m;
+    ac.c2 = p.c2;
+    ac.clight = p.clight;
+    ac.vclamp = p.vclamp;
+    for (int k = 0; k < 3; ++k) {
+        ac.yc[k] = 0.0;
+        ac.yd[k] = 0.0;
+    }
+    d3 r = mk3(p.x[i], p.y[i], p.z[i]);
+    d3 v = mk3(p.vx[i], p.vy[i], p.vz[i]);
+    switch (bd.algorithm) {
+        case PP_ALG_BORIS: batch_steps<PP_ALG_BORIS, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac); break;
+        case PP_ALG_VAY: batch_steps<PP_ALG_VAY, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac); break;
+        case PP_ALG_VAY_REL:
+            batch_steps<PP_ALG_VAY_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            break;
+        case PP_ALG_BORIS_REL:
+            batch_steps<PP_ALG_BORIS_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            break;
+        case PP_ALG_RK4: batch_steps<PP_ALG_RK4, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac); break;
+        case PP_ALG_RK4_REL:
+            batch_steps<PP_ALG_RK4_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            break;
+        case PP_ALG_LEAPFROG:
+            batch_steps<PP_ALG_LEAPFROG, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            break;
+        default: return;
+    }
+    p.x[i] = r.x;
+    p.y[i] = r.y;
+    p.z[i] = r.z;
+    p.vx[i] = v.x;
+    p.vy[i] = v.y;
+    p.vz[i] = v.z;
+}
+
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int block, hipStream_t s) {
+    const dim3 g((unsigned)p.ntiles), b(block);
+    const bool mixed = prec == PP_PREC_FP32_MIXED;
+    if (mixed) {
+        if (periodic) hipLaunchKernelGGL((k_batched<true, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_batched<true, false>), g, b, 0, s, p);
+    } else {
+        if (periodic) hipLaunchKernelGGL((k_batched<false, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_batched<false, false>), g, b, 0, s, p);
+    }
+    return hipGetLastError();
+}
 
 #define PP_FAST_CASES(KERNEL)                                                                             \
     switch (alg) {                                                                                        \

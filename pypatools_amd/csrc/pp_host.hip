@@ -64,6 +64,13 @@ int pp_struct_sizes(int64_t* field_size, int64_t* push_args_size) {
     return PP_OK;
 }
 
+int pp_batch_struct_sizes(int64_t* bunch_size, int64_t* batch_args_size) {
+    if (!bunch_size || !batch_args_size) return fail(PP_ERR_INVALID, "null argument");
+    *bunch_size = (int64_t)sizeof(pp_bunch);
+    *batch_args_size = (int64_t)sizeof(pp_batch_args);
+    return PP_OK;
+}
+
 int pp_get_constants(double* out, int32_t n) {
     if (!out || n < 9) return fail(PP_ERR_INVALID, "need room for 9 constants");
     out[0] = kClight;
@@ -373,6 +380,25 @@ static int make_field(const pp_field& f, int precision, FieldDev& out) {
     return PP_OK;
 }
 
+static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, HotMap& h) {
+    memset(&h, 0, sizeof(h));
+    h.coef = coef;
+    h.bytes = (uint32_t)bytes;
+    h.nc0 = m.nc[0];
+    h.nc1 = m.nc[1];
+    h.nc2 = m.nc[2];
+    h.id0 = m.inv_d[0];
+    h.id1 = m.inv_d[1];
+    h.id2 = m.inv_d[2];
+    h.c0 = -m.lo[0] * m.inv_d[0];
+    h.c1 = -m.lo[1] * m.inv_d[1];
+    h.c2 = -m.lo[2] * m.inv_d[2];
+    h.inv_nz = m.inv_nz;
+    h.sE = 1.0;
+    h.sB = 1.0;
+    h.fill = m.fill;
+}
+
 // The hot kernels cover the common case: E and B from one uniform 3-D map built
 // with E in components 0..2 and B in 3..5, no RF / offset, every particle active,
 // one dt, a single-gather algorithm or RK4, fast or mixed precision.
@@ -406,22 +432,9 @@ static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& 
     hp.rec_every = p.rec_every;
     hp.n_records = p.n_records;
     hp.kick_only = p.kick_only;
-    HotMap& h = hp.m;
-    h.coef = coef;
-    h.bytes = (uint32_t)bytes;
-    h.nc0 = m.nc[0];
-    h.nc1 = m.nc[1];
-    h.nc2 = m.nc[2];
-    h.id0 = m.inv_d[0];
-    h.id1 = m.inv_d[1];
-    h.id2 = m.inv_d[2];
-    h.c0 = -m.lo[0] * m.inv_d[0];
-    h.c1 = -m.lo[1] * m.inv_d[1];
-    h.c2 = -m.lo[2] * m.inv_d[2];
-    h.inv_nz = m.inv_nz;
-    h.sE = E.has_scale2 ? E.scale1 * E.scale2 : E.scale1;
-    h.sB = B.has_scale2 ? B.scale1 * B.scale2 : B.scale1;
-    h.fill = m.fill;
+    fill_hot_map(m, coef, bytes, hp.m);
+    hp.m.sE = E.has_scale2 ? E.scale1 * E.scale2 : E.scale1;
+    hp.m.sB = B.has_scale2 ? B.scale1 * B.scale2 : B.scale1;
     return true;
 }
 
@@ -518,6 +531,134 @@ int pp_push(const pp_push_args* a, pp_stream s) {
     else
         e = pp_fast::launch_push(a->algorithm, a->precision, p, a->block_size, stream_of(s));
     if (e != hipSuccess) return hip_fail(e, "push launch");
+    return PP_OK;
+}
+
+int pp_batch_tiles(const pp_bunch* bunches, int32_t nbunch, int32_t tile, int32_t* tile_bunch, int64_t* tile_start,
+                   int64_t* ntiles) {
+    if (!bunches || nbunch < 0 || tile < 64 || tile > 1024 || (tile % 64) || !ntiles)
+        return fail(PP_ERR_INVALID, "bad tile arguments");
+    int64_t t = 0;
+    for (int32_t b = 0; b < nbunch; ++b) {
+        if (bunches[b].count < 0 || bunches[b].offset < 0) return fail(PP_ERR_INVALID, "bunch %d: bad range", b);
+        for (int64_t o = 0; o < bunches[b].count; o += tile, ++t) {
+            if (tile_bunch) tile_bunch[t] = b;
+            if (tile_start) tile_start[t] = bunches[b].offset + o;
+        }
+    }
+    *ntiles = t;
+    return PP_OK;
+}
+
+int pp_push_batched(const pp_batch_args* a, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!a) return fail(PP_ERR_INVALID, "null args");
+    if (a->precision != PP_PREC_FP64_FAST && a->precision != PP_PREC_FP32_MIXED)
+        return fail(PP_ERR_INVALID, "batched pushes run in PP_PREC_FP64_FAST or PP_PREC_FP32_MIXED");
+    if (a->nmaps < 1 || a->nmaps > PP_MAX_BATCH_MAPS) return fail(PP_ERR_INVALID, "nmaps must be 1..%d",
+                                                                  PP_MAX_BATCH_MAPS);
+    if (a->ntiles < 0 || a->nsteps < 0 || a->step0 < 0 || a->n < 0) return fail(PP_ERR_INVALID, "negative size");
+    if (a->ntiles == 0 || a->nsteps == 0) return PP_OK;
+    if (a->ntiles > 0x7FFFFFFF) return fail(PP_ERR_INVALID, "too many tiles");
+    if (a->block_size != 64 && a->block_size != 128 && a->block_size != 256)
+        return fail(PP_ERR_INVALID, "block_size must be 64, 128 or 256 (the tile size)");
+    if (!a->x || !a->y || !a->z || !a->vx || !a->vy || !a->vz || !a->bunches || !a->tile_bunch || !a->tile_start)
+        return fail(PP_ERR_INVALID, "null device pointer");
+    BatchParams p;
+    memset(&p, 0, sizeof(p));
+    const bool mixed = a->precision == PP_PREC_FP32_MIXED;
+    int periodic = -1;
+    for (int k = 0; k < a->nmaps; ++k) {
+        const pp_map* mp = a->maps[k];
+        if (!mp) return fail(PP_ERR_INVALID, "map %d is null", k);
+        const MapDesc& m = mp->d;
+        if (mp->device != g_device) return fail(PP_ERR_INVALID, "map %d lives on another device", k);
+        if (m.ncomp != 6 || !m.uniform || m.n[0] < 2 || m.n[1] < 2 || m.n[2] < 2)
+            return fail(PP_ERR_UNSUPPORTED, "map %d: batched pushes need 6-component uniform 3-D maps", k);
+        const void* coef = mixed ? (const void*)m.coef32 : (const void*)m.coef64;
+        if (!coef) return fail(PP_ERR_UNSUPPORTED, "map %d has no coefficient table for this precision", k);
+        const uint64_t bytes = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2] * 48 * (mixed ? 4 : 8);
+        if (bytes >= 0x7FFFFFFFull) return fail(PP_ERR_UNSUPPORTED, "map %d too large for 32-bit offsets", k);
+        if (periodic < 0) periodic = m.periodic;
+        else if (periodic != m.periodic) return fail(PP_ERR_UNSUPPORTED, "maps must all be periodic or all not");
+        fill_hot_map(m, coef, bytes, p.maps[k]);
+    }
+    p.x = a->x; p.y = a->y; p.z = a->z;
+    p.vx = a->vx; p.vy = a->vy; p.vz = a->vz;
+    p.bunches = a->bunches;
+    p.tile_bunch = a->tile_bunch;
+    p.tile_start = a->tile_start;
+    p.ntiles = a->ntiles;
+    p.n = a->n;
+    p.step0 = a->step0;
+    p.step0_dev = a->step0_dev;
+    p.nsteps = a->nsteps;
+    p.c2 = kClight * kClight;
+    p.clight = kClight;
+    p.vclamp = 0.9999 * kClight;
+    p.nmaps = a->nmaps;
+    p.nbunch = a->nbunch;
+    HIP_TRY(pp_fast::launch_batched(a->precision, periodic == 1, p, a->block_size, stream_of(s)), "batched launch");
+    return PP_OK;
+}
+
+int pp_graph_begin(pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!s) return fail(PP_ERR_INVALID, "graph capture needs a created stream");
+    HIP_TRY(hipStreamBeginCapture(static_cast<hipStream_t>(s), hipStreamCaptureModeThreadLocal), "begin capture");
+    return PP_OK;
+}
+
+int pp_graph_end(pp_stream s, pp_graph* g) {
+    if (!s || !g) return fail(PP_ERR_INVALID, "null argument");
+    hipGraph_t graph;
+    HIP_TRY(hipStreamEndCapture(static_cast<hipStream_t>(s), &graph), "end capture");
+    hipGraphExec_t exec;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+    *g = exec;
+    return PP_OK;
+}
+
+int pp_graph_launch(pp_graph g, pp_stream s) {
+    if (!g) return fail(PP_ERR_INVALID, "null graph");
+    HIP_TRY(hipGraphLaunch(static_cast<hipGraphExec_t>(g), stream_of(s)), "graph launch");
+    return PP_OK;
+}
+
+int pp_graph_destroy(pp_graph g) {
+    if (g) HIP_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(g)), "graph destroy");
+    return PP_OK;
+}
+
+int pp_add_i64(int64_t* dptr, int64_t value, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!dptr) return fail(PP_ERR_INVALID, "null pointer");
+    HIP_TRY(pp_strict::launch_add_i64(dptr, value, stream_of(s)), "add kernel");
+    return PP_OK;
+}
+
+int pp_sort_state_grouped(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
+                          int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!a || !ids || !map || !group) return fail(PP_ERR_INVALID, "null argument");
+    if (a->n < 0 || a->n > INT32_MAX) return fail(PP_ERR_INVALID, "sort supports 0 <= n < 2^31");
+    if (a->n == 0) return PP_OK;
+    const MapDesc& m = map->d;
+    const uint64_t ncell = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2];
+    if (group_shift < 0 || group_shift > 31 || ((uint64_t)1 << group_shift) < ncell)
+        return fail(PP_ERR_INVALID, "group_shift %d leaves no room for %llu cells", group_shift,
+                    (unsigned long long)ncell);
+    size_t need = 0;
+    HIP_TRY(pp_sort::workspace_bytes(a->n, &need), "sort workspace query");
+    if (!ws || ws_bytes < need) return fail(PP_ERR_INVALID, "sort workspace of %zu bytes needed", need);
+    double* arrs[6] = {a->x, a->y, a->z, a->vx, a->vy, a->vz};
+    HIP_TRY(pp_sort::sort_state(m, arrs, ids, a->n, ws, ws_bytes, stream_of(s), group, group_shift), "grouped sort");
     return PP_OK;
 }
 

@@ -89,6 +89,22 @@ struct HotParams {
     HotMap m;
 };
 
+struct BatchParams {
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    const pp_bunch* bunches;
+    const int32_t* tile_bunch;
+    const int64_t* tile_start;
+    int64_t ntiles;
+    int64_t n;              // particles in the state (bounds)
+    int64_t step0, nsteps;
+    const int64_t* step0_dev;
+    double c2, clight, vclamp;
+    int32_t nmaps;
+    int32_t nbunch;
+    HotMap maps[PP_MAX_BATCH_MAPS];
+};
+
 // Host-side map object behind the opaque pp_map handle.
 struct pp_map {
     MapDesc d;
@@ -116,12 +132,14 @@ hipError_t launch_batch_kernel(const BatchArgs& args, hipStream_t s);
 hipError_t launch_util(int which, double* a, double* b, double* c, double* aos, int64_t m, double c2,
                        hipStream_t s);
 hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
+hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t s);
 hipError_t launch_build_coef(const MapDesc& d, double* c64, float* c32, hipStream_t s);
 }  // namespace pp_strict
 
 namespace pp_fast {
 hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStream_t s);
 hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int block, hipStream_t s);
+hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int block, hipStream_t s);
 hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int64_t m, double rf, double* out,
                              hipStream_t s);
 }  // namespace pp_fast
@@ -129,7 +147,7 @@ hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int
 namespace pp_sort {
 hipError_t workspace_bytes(int64_t n, size_t* bytes);
 hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,
-                      hipStream_t s);
+                      hipStream_t s, const int32_t* group = nullptr, int group_shift = 0);
 hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes, hipStream_t s);
 hipError_t iota(int32_t* ids, int64_t n, hipStream_t s);
 }  // namespace pp_sort

@@ -85,6 +85,10 @@ __global__ void __launch_bounds__(256) k_util(int which, double* a, double* b, d
     }
 }
 
+__global__ void k_add_i64(int64_t* p, int64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
 __global__ void __launch_bounds__(256) k_fill(double* p, double v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -161,6 +165,11 @@ hipError_t launch_util(int which, double* a, double* b, double* c, double* aos, 
 
 hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_i64, dim3(1), dim3(64), 0, s, p, v);
     return hipGetLastError();
 }
 
