@@ -43,8 +43,10 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--steps-per-launch", type=int, default=0)
-    ap.add_argument("--sort-every", type=int, default=2000, help="steps between cell sorts (0: never)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"])
+    ap.add_argument("--sort-every", type=int, default=250, help="steps between cell sorts (0: never)")
+    ap.add_argument("--refine-bits", type=int, default=3, help="sub-cell sort bins along the motion (2^bits)")
+    ap.add_argument("--block", type=int, default=0, help="threads per block (0: library default 256)")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--bunches", type=int, default=512)
     ap.add_argument("--c4-scale", type=float, default=1.0, help="scale C4 bunch sizes (1.0 = the C4 config)")
     ap.add_argument("--autotune", action="store_true")
@@ -139,6 +141,103 @@ def bench_c4(args):
     print(json.dumps(out), flush=True)
 
 
+def _gamma(v):
+    from pypatools_amd.constants import CLIGHT
+    b2 = np.sum(v * v, axis=1) / CLIGHT ** 2
+    return 1.0 / np.sqrt(1.0 - b2)
+
+
+def _track_rate(tracker, st, fp, nsteps, dt, reps=1, t0=0.0):
+    """Median particle-steps/s of `reps` runs of nsteps from the resident initial state `st`."""
+    from pypatools_amd.device import synchronize
+    from pypatools_amd.engine import ParticleState
+    rates, last = [], None
+    for _ in range(reps):
+        s = ParticleState(st.n)
+        for d, q in zip((s.x, s.y, s.z, s.vx, s.vy, s.vz), (st.x, st.y, st.z, st.vx, st.vy, st.vz)):
+            d.copy_from(q)
+        synchronize()
+        t = time.perf_counter()
+        tracker.run_resident(s, fp, nsteps, dt, step0=0, rf=tracker._rf(fp, t0, dt, nsteps))
+        synchronize()
+        rates.append(st.n * nsteps / (time.perf_counter() - t))
+        last = s
+    last.unsort()
+    return float(np.median(rates)), last.to_host()
+
+
+def bench_c3(args):
+    """C3: RK4_rel through a TM010-like RF map (E cos, B sin), 4e6 electrons, 5000 steps; FP32-mixed vs FP64."""
+    import pypatools_amd as pa
+    from pypatools_amd.engine import FieldPair, ParticleState, Tracker
+    from pypatools_amd.workloads import c3_bunch, c3_map
+    M = args.particles if args.particles != 1_000_000 else 4_000_000
+    grid, E, B, w = c3_map()
+    r0, v0, phase = c3_bunch(M)
+    N, dt = 5000, 2e-13
+    fe = pa.RFField(pa.Field.from_arrays(grid, E), w, phase, "cos")
+    fb = pa.RFField(pa.Field.from_arrays(grid, B), w, phase, "sin")
+    fp = FieldPair(fe, fb)
+    qm = pa.IonSpecies("electron").q_over_m
+    st0 = ParticleState.from_host(r0, v0)
+    res = {}
+    for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
+        tr = Tracker("rk4_rel", prec, qm, sort_every=args.sort_every, refine_bits=args.refine_bits)
+        res[prec] = _track_rate(tr, st0, fp, N, dt, reps=args.steps if prec == args.precision else 1)
+    value, (rm, vm) = res[args.precision]
+    extra = {}
+    if "fp64" in res:
+        r64, v64 = res["fp64"][1]
+        path = np.linalg.norm(v0, axis=1) * N * dt
+        dr = np.linalg.norm(rm - r64, axis=1)
+        g64, gm = _gamma(v64), _gamma(vm)
+        extra = {"fp64_particle_steps_per_s": res["fp64"][0], "pos_err_rel_path_max": float(np.max(dr / path)),
+                 "pos_err_max_m": float(np.max(dr)), "energy_err_rel_max": float(np.max(np.abs(gm - g64) / (g64 - 1))),
+                 "speedup_over_fp64": value / res["fp64"][0]}
+    extra["mean_kinetic_gain_keV"] = float(np.mean(_gamma(vm) - _gamma(v0)) * 510.99895069)
+    out = {"metric": "particle-steps/sec on 1 MI355X (C3 RK4_rel through an RF map)", "value": value,
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+           "ms_per_step": 1e3 * M * N / value, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64" if args.precision == "fp32_mixed" else "f64", "data": "synthetic",
+           "config": {"workload": f"C3: 96x96x256 TM010-like map (E0 10 MV/m, 100 MHz, E cos / B sin, phase "
+                                  f"{phase:.3f}), {M} electrons at 100 keV in a 3 mm disk, dt 2e-13 s, {N} steps, "
+                                  "rk4_rel", "precision": args.precision, "particles": M},
+           "extra": extra}
+    print(json.dumps(out), flush=True)
+
+
+def bench_c1(args):
+    """C1: FP64 Boris in a uniform 1 T field, 1e4 protons x 1000 steps; error vs the closed-form gyro-orbit."""
+    import pypatools_amd as pa
+    from pypatools_amd.engine import FieldPair, ParticleState, Tracker
+    from pypatools_amd.workloads import c1
+    w = c1()
+    r0, v0, N, dt = w["r0"], w["v0"], w["nsteps"], w["dt"]
+    fp = FieldPair(pa.Field(dim=0, field=dict(zip("xyz", w["E"]))), pa.Field(dim=0, field=dict(zip("xyz", w["B"]))))
+    qm = pa.IonSpecies("proton").q_over_m
+    st0 = ParticleState.from_host(r0, v0)
+    value, (r, v) = _track_rate(Tracker("boris", "fp64", qm), st0, fp, N, dt, reps=args.steps)
+    # closed form of the staggered Boris map in uniform B along z: v rotates by -theta per step,
+    # theta = 2 atan(qm B dt / 2); r_N = r_0 + dt sum_{n=1..N} R^n v_0
+    th = 2.0 * np.arctan(qm * 1.0 * dt / 2.0)
+    n = np.arange(1, N + 1)
+    c, s = np.cos(n * th).sum(), np.sin(n * th).sum()
+    rx = r0[:, 0] + dt * (c * v0[:, 0] + s * v0[:, 1])
+    ry = r0[:, 1] + dt * (-s * v0[:, 0] + c * v0[:, 1])
+    rz = r0[:, 2] + dt * N * v0[:, 2]
+    exact = np.column_stack([rx, ry, rz])
+    path = np.linalg.norm(v0, axis=1) * N * dt
+    err = float(np.max(np.linalg.norm(r - exact, axis=1) / path))
+    out = {"metric": "particle-steps/sec on 1 MI355X (C1 plain FP64 Boris, uniform B)", "value": value,
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+           "ms_per_step": 1e3 * len(r0) * N / value, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "C1: 1e4 protons at 1 MeV, isotropic, uniform B = 1 T along z, E = 0, "
+                                  "dt 1e-11 s, 1000 steps, boris (FP64)", "precision": "fp64"},
+           "extra": {"pos_err_rel_path_max_vs_closed_form": err, "theta_per_step": th}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_c5(args):
     """C5: Poisson arrivals, continuous batching, p50/p99 latency at a load set from measured capacity."""
     import pypatools_amd as pa
@@ -195,6 +294,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine shards by particle "
               "index across GPUs by design but this build measures one GPU's share", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c1":
+        return bench_c1(args)
+    if args.workload == "c3":
+        return bench_c3(args)
     if args.workload == "c4":
         return bench_c4(args)
     if args.workload == "c5":
@@ -214,7 +317,8 @@ def main():
     spl = args.steps_per_launch or None
 
     def segment(prec, state, step0, timer=None):
-        tr = Tracker("boris", prec, qm, sort_every=args.sort_every)
+        tr = Tracker("boris", prec, qm, block_size=args.block, sort_every=args.sort_every,
+                     refine_bits=args.refine_bits, refine_axis=2)
         return tr.run_resident(state, fp, N, dt, step0=step0, steps_per_launch=spl, timer=timer)
 
     st0 = ParticleState.from_host(w["r0"], w["v0"])    # resident initial bunch
@@ -281,7 +385,7 @@ def main():
     avg_ms = float(np.mean(kern_ms))
     work = args.steps * M * N
     value = work / el
-    # roofline of the dominant kernel (k_push_mixed<boris>): algorithmic gather bytes per launch / launch time
+    # roofline of the dominant kernel (k_hot<boris>): algorithmic gather bytes per launch / launch time
     units = M * steps_per_launch
     achieved = GATHER_BYTES[args.precision] * units / (avg_ms * 1e-3) / 1e9
     traffic = None
@@ -299,12 +403,14 @@ def main():
                    "precision": args.precision, "particles": M, "time_steps_per_bench_step": N},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"k_push_mixed<boris> ({steps_per_launch} steps/launch, avg {avg_ms:.2f} ms)",
+                     "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true> (boris, "
+                               f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.2f} ms)",
                      "algorithmic_bytes_per_particle_step": GATHER_BYTES[args.precision],
                      "gflops": FLOPS_PER_STEP * units / (avg_ms * 1e-3) / 1e9},
         "extra": extra,
     }
     out["config"]["sort_every"] = args.sort_every
+    out["config"]["refine_bits"] = args.refine_bits
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
         out["extra"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

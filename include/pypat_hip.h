@@ -281,6 +281,13 @@ int pp_sort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t w
    bunch keeps its slot range; the map is `map`. */
 int pp_sort_state_grouped(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
                           int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
+/* General form: key = ((cell << refine_bits) | sub-cell bin along refine_axis) | (group << group_shift).
+   Sub-cell bins along the direction of motion keep a wavefront's particles in one
+   cell for more steps.  map: NULL = efield's map, else bfield's; group may be NULL.
+   The key must fit 32 bits. */
+int pp_sort_state_ex(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
+                     int32_t refine_axis, int32_t refine_bits, int32_t* ids, void* workspace, size_t ws_bytes,
+                     pp_stream s);
 int pp_unsort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
 int pp_iota_i32(int32_t* ids, int64_t n, pp_stream s);
 

@@ -137,6 +137,8 @@ SIGNATURES = {
     "pp_push_batched": (c_int32, [c_void_p, c_void_p]),
     "pp_sort_state_grouped": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
                                         ctypes.c_size_t, c_void_p]),
+    "pp_sort_state_ex": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                   c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_graph_begin": (c_int32, [c_void_p]),
     "pp_graph_end": (c_int32, [c_void_p, ctypes.POINTER(c_void_p)]),
     "pp_graph_launch": (c_int32, [c_void_p, c_void_p]),

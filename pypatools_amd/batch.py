@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Stream, synchronize
-from .engine import U_STATE, _fused_map
+from .engine import DEFAULT_REFINE_BITS, U_STATE, _fused_map, motion_axis
 from .interpolators import DeviceFieldMap
 
 BATCH_ALGORITHMS = ("boris", "vay", "vay_rel", "boris_rel", "rk4", "rk4_rel", "leapfrog")
@@ -123,9 +123,13 @@ class BatchedTracker:
         m0 = self.maps[0]
         ncell = int(np.prod([max(s - 1, 1) for s in m0.shape[:2]]) * (m0.shape[2] if m0.periodic_z
                                                                       else max(m0.shape[2] - 1, 1)))
-        self.group_shift = max(1, int(np.ceil(np.log2(max(ncell, 2)))))
-        if len(self.bunches) > 2 ** (32 - self.group_shift):
+        cell_bits = max(1, int(np.ceil(np.log2(max(ncell, 2)))))
+        group_bits = max(1, int(np.ceil(np.log2(max(len(self.bunches), 2)))))
+        self.refine_bits = max(0, min(DEFAULT_REFINE_BITS, 32 - cell_bits - group_bits))
+        self.group_shift = cell_bits + self.refine_bits
+        if self.group_shift + group_bits > 32:
             self.sort_every = 0     # no room in the key for the bunch index
+        self.refine_axis = motion_axis(np.concatenate([b.v0[:1000] for b in self.bunches])) if self.bunches else 2
         self.state.ensure_ids()
         self.max_steps = max((b.nsteps for b in self.bunches), default=0)
         self.step = 0
@@ -152,8 +156,8 @@ class BatchedTracker:
         st = self.state
         pa = st._state_args()
         ws = st._workspace()
-        _lib.call("pp_sort_state_grouped", pa, self.maps[0].handle, self.group.ptr, self.group_shift, st.ids.ptr,
-                  ws.ptr, ws.nbytes, None)
+        _lib.call("pp_sort_state_ex", pa, self.maps[0].handle, self.group.ptr, self.group_shift, self.refine_axis,
+                  self.refine_bits, st.ids.ptr, ws.ptr, ws.nbytes, None)
         st.sorted = True
 
     # ---- running ---------------------------------------------------------------

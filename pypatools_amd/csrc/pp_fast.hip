@@ -363,6 +363,21 @@ DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& t
     return (i * (uint32_t)m.nc1 + j) * (uint32_t)m.nc2 + k;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Constant-address-space view of the coefficient table: loads at a wave-uniform
+// address become scalar loads (s_load_dwordx16) into SGPRs.
+typedef const __attribute__((address_space(4))) float* cfloat_p;
+
+DEV float ceval_s(cfloat_p q, float tx, float ty, float tz) {
+    const float p1 = fmaf(ty, q[4], q[1]);
+    const float p2 = fmaf(tx, q[7], q[6]);
+    const float p3 = fmaf(tx, q[5], q[3]);
+    const float p4 = fmaf(ty, p2, p3);
+    const float r = fmaf(ty, q[2], fmaf(tx, p1, q[0]));
+    return fmaf(tz, p4, r);
+}
+#endif
+
 template <bool PERIODIC>
 struct HotFE32 {
     const HotMap& m;
@@ -371,12 +386,25 @@ struct HotFE32 {
         double tx, ty, tz;
         bool ok;
         const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
-        const uint32_t base = cell * (48u * 4u);
         const float fx = (float)tx, fy = (float)ty, fz = (float)tz;
         float v[6];
+#if defined(__HIP_DEVICE_COMPILE__)
+        // A wavefront whose particles all sit in one cell (the common case after a
+        // cell sort) reads the cell's 192 B once through the scalar cache; other
+        // wavefronts gather per lane through the vector memory path.
+        const uint32_t c0 = __builtin_amdgcn_readfirstlane(cell);
+        if (__builtin_amdgcn_ballot_w64(cell != c0) == 0) {
+            const cfloat_p q = (cfloat_p)m.coef + (size_t)c0 * 48u;
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-            v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), fx, fy, fz);
+            for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, fx, fy, fz);
+        } else
+#endif
+        {
+            const uint32_t base = cell * (48u * 4u);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), fx, fy, fz);
+        }
         const float fl = (float)m.fill, sE = (float)m.sE, sB = (float)m.sB;
         E = mkf(sE * (ok ? v[0] : fl), sE * (ok ? v[1] : fl), sE * (ok ? v[2] : fl));
         B = mkf(sB * (ok ? v[3] : fl), sB * (ok ? v[4] : fl), sB * (ok ? v[5] : fl));

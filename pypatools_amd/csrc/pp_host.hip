@@ -642,24 +642,41 @@ int pp_add_i64(int64_t* dptr, int64_t value, pp_stream s) {
     return PP_OK;
 }
 
-int pp_sort_state_grouped(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
-                          int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
+static const MapDesc* sort_map(const pp_push_args* a);
+
+int pp_sort_state_ex(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
+                     int32_t refine_axis, int32_t refine_bits, int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
     int rc = ensure_init();
     if (rc) return rc;
-    if (!a || !ids || !map || !group) return fail(PP_ERR_INVALID, "null argument");
+    if (!a || !ids) return fail(PP_ERR_INVALID, "null argument");
     if (a->n < 0 || a->n > INT32_MAX) return fail(PP_ERR_INVALID, "sort supports 0 <= n < 2^31");
     if (a->n == 0) return PP_OK;
-    const MapDesc& m = map->d;
+    if (!a->x || !a->y || !a->z || !a->vx || !a->vy || !a->vz) return fail(PP_ERR_INVALID, "null state pointer");
+    const MapDesc* mp = map ? &map->d : sort_map(a);
+    if (!mp) return PP_OK;
+    const MapDesc& m = *mp;
+    if (refine_axis < 0 || refine_axis > 2 || refine_bits < 0 || refine_bits > 8)
+        return fail(PP_ERR_INVALID, "refine_axis must be 0..2 and refine_bits 0..8");
     const uint64_t ncell = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2];
-    if (group_shift < 0 || group_shift > 31 || ((uint64_t)1 << group_shift) < ncell)
-        return fail(PP_ERR_INVALID, "group_shift %d leaves no room for %llu cells", group_shift,
-                    (unsigned long long)ncell);
+    const uint64_t span = ncell << refine_bits;
+    if (group && (group_shift < 0 || group_shift > 31 || ((uint64_t)1 << group_shift) < span))
+        return fail(PP_ERR_INVALID, "group_shift %d leaves no room for %llu keys", group_shift,
+                    (unsigned long long)span);
+    if (span > 0xFFFFFFFFull) return fail(PP_ERR_UNSUPPORTED, "map too large for 32-bit sort keys");
     size_t need = 0;
     HIP_TRY(pp_sort::workspace_bytes(a->n, &need), "sort workspace query");
     if (!ws || ws_bytes < need) return fail(PP_ERR_INVALID, "sort workspace of %zu bytes needed", need);
     double* arrs[6] = {a->x, a->y, a->z, a->vx, a->vy, a->vz};
-    HIP_TRY(pp_sort::sort_state(m, arrs, ids, a->n, ws, ws_bytes, stream_of(s), group, group_shift), "grouped sort");
+    HIP_TRY(pp_sort::sort_state(m, arrs, ids, a->n, ws, ws_bytes, stream_of(s), group, group_shift, refine_axis,
+                                refine_bits),
+            "sort");
     return PP_OK;
+}
+
+int pp_sort_state_grouped(const pp_push_args* a, const pp_map* map, const int32_t* group, int32_t group_shift,
+                          int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
+    if (!map || !group) return fail(PP_ERR_INVALID, "null argument");
+    return pp_sort_state_ex(a, map, group, group_shift, 2, 0, ids, ws, ws_bytes, s);
 }
 
 static const MapDesc* sort_map(const pp_push_args* a) {

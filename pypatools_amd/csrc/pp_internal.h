@@ -147,7 +147,8 @@ hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int
 namespace pp_sort {
 hipError_t workspace_bytes(int64_t n, size_t* bytes);
 hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,
-                      hipStream_t s, const int32_t* group = nullptr, int group_shift = 0);
+                      hipStream_t s, const int32_t* group = nullptr, int group_shift = 0, int refine_axis = 2,
+                      int refine_bits = 0);
 hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes, hipStream_t s);
 hipError_t iota(int32_t* ids, int64_t n, hipStream_t s);
 }  // namespace pp_sort

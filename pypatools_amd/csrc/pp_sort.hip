@@ -25,16 +25,30 @@ __device__ __forceinline__ uint32_t key_axis(const MapDesc& m, int a, double x, 
     return (uint32_t)f;
 }
 
+// Sub-cell position along one axis, in 2^bits bins (0 if the axis is inactive).
+__device__ __forceinline__ uint32_t sub_axis(const MapDesc& m, int a, double x, int bits) {
+    if (bits <= 0 || m.n[a] <= 1) return 0;
+    const double u = (x - m.lo[a]) * m.inv_d[a];
+    double f = (u - floor(u)) * (double)(1u << bits);
+    f = fmin(fmax(f, 0.0), (double)((1u << bits) - 1));
+    return (uint32_t)f;
+}
+
 __global__ void __launch_bounds__(256) k_keys(const MapDesc m, const double* __restrict__ x,
                                               const double* __restrict__ y, const double* __restrict__ z, int64_t n,
-                                              const int32_t* __restrict__ group, int group_shift,
-                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ slots) {
+                                              const int32_t* __restrict__ group, int group_shift, int refine_axis,
+                                              int refine_bits, uint32_t* __restrict__ keys,
+                                              uint32_t* __restrict__ slots) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t a = key_axis(m, 0, x[i], false);
     const uint32_t b = key_axis(m, 1, y[i], false);
     const uint32_t c = key_axis(m, 2, z[i], m.periodic != 0);
     uint32_t k = (a * (uint32_t)m.nc[1] + b) * (uint32_t)m.nc[2] + c;
+    if (refine_bits > 0) {
+        const double pos = refine_axis == 0 ? x[i] : (refine_axis == 1 ? y[i] : z[i]);
+        k = (k << refine_bits) | sub_axis(m, refine_axis, pos, refine_bits);
+    }
     if (group) k |= (uint32_t)group[i] << group_shift;
     keys[i] = k;
     slots[i] = (uint32_t)i;
@@ -89,7 +103,7 @@ static int key_bits(const MapDesc& m) {
 }
 
 hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,
-                      hipStream_t s, const int32_t* group, int group_shift) {
+                      hipStream_t s, const int32_t* group, int group_shift, int refine_axis, int refine_bits) {
     size_t need = 0;
     hipError_t e = workspace_bytes(n, &need);
     if (e != hipSuccess) return e;
@@ -104,8 +118,8 @@ hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n
     char* cub_tmp = w + 4 * q + align256(8 * (size_t)n);
     size_t cub_bytes = ws_bytes - (4 * q + align256(8 * (size_t)n));
     hipLaunchKernelGGL(pp_sortk::k_keys, dim3(grid_for(n)), dim3(256), 0, s, m, arrs[0], arrs[1], arrs[2], n, group,
-                       group_shift, keys_in, slots_in);
-    const int bits = group ? 32 : key_bits(m);
+                       group_shift, refine_axis, refine_bits, keys_in, slots_in);
+    const int bits = group ? 32 : (key_bits(m) + refine_bits > 32 ? 32 : key_bits(m) + refine_bits);
     e = hipcub::DeviceRadixSort::SortPairs(cub_tmp, cub_bytes, keys_in, keys_out, slots_in, perm, (int)n, 0, bits,
                                            s);
     if (e != hipSuccess) return e;

@@ -32,8 +32,16 @@ RK4_FAMILY = ("rk4", "rk4_rel")
 # particle-steps per fused launch (keeps single launches around a second or less)
 DEFAULT_LAUNCH_WORK = 2 ** 30
 # steps between cell sorts of the state when a field is a gridded map
-DEFAULT_SORT_EVERY = 2000
+DEFAULT_SORT_EVERY = 250
 SORT_MIN_PARTICLES = 4096
+# sub-cell bins (2^bits) along the direction of motion in the sort key
+DEFAULT_REFINE_BITS = 3
+
+
+def motion_axis(v):
+    """Axis of the largest mean |velocity| component of an (M, 3) array."""
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    return int(np.argmax(np.abs(v[: min(len(v), 100000)]).mean(axis=0))) if len(v) else 2
 
 
 class ParticleState:
@@ -103,13 +111,19 @@ class ParticleState:
             a.efield, a.bfield = fp.tE, fp.tB
         return a
 
-    def sort(self, fp):
-        """Reorder by map cell (gather locality); a no-op when neither field is a map."""
+    def sort(self, fp, refine_axis=2, refine_bits=0):
+        """Reorder by map cell (gather locality); a no-op when neither field is a map.
+
+        refine_bits > 0 also orders particles inside a cell in 2^bits bins along
+        refine_axis (the direction of motion), so a wavefront stays in one cell
+        for more steps and reads the map through the scalar cache.
+        """
         if self.n == 0:
             return
         ids = self.ensure_ids()
         ws = self._workspace()
-        _lib.call("pp_sort_state", self._state_args(fp), ids.ptr, ws.ptr, ws.nbytes, None)
+        _lib.call("pp_sort_state_ex", self._state_args(fp), None, None, 0, int(refine_axis), int(refine_bits),
+                  ids.ptr, ws.ptr, ws.nbytes, None)
         self.sorted = True
 
     def reset_order(self):
@@ -207,7 +221,7 @@ class Tracker:
     """One track of n particles through (efield, bfield) on the device."""
 
     def __init__(self, algorithm, precision, q_over_m, block_size=0, launch_work=DEFAULT_LAUNCH_WORK,
-                 sort_every=DEFAULT_SORT_EVERY):
+                 sort_every=DEFAULT_SORT_EVERY, refine_bits=DEFAULT_REFINE_BITS, refine_axis=2):
         if algorithm not in _lib.ALG_IDS:
             raise ValueError(f"unknown algorithm '{algorithm}'")
         if precision not in _lib.PRECISIONS:
@@ -218,6 +232,8 @@ class Tracker:
         self.block_size = int(block_size)
         self.launch_work = int(launch_work)
         self.sort_every = int(sort_every)   # steps between cell sorts (0: never) for map fields
+        self.refine_bits = int(refine_bits)
+        self.refine_axis = int(refine_axis)
 
     # -- kernel launch helpers ---------------------------------------------------
     def _args(self, st, fp, dt, nsteps, step0, tE=None, tB=None, rf=None, hist=None, kick=False, dt_pp=None):
@@ -277,6 +293,7 @@ class Tracker:
             return empty, empty.copy(), np.ones(0, dtype=bool)
         st = ParticleState.from_host(r0, v0)
         fp = FieldPair(efield, bfield)
+        self.refine_axis = motion_axis(v0)
         elec = electrode_assembly
         active = np.ones(M, dtype=bool)
         if elec is not None:
@@ -336,7 +353,7 @@ class Tracker:
         since = None
         while step < end:
             if sort and (since is None or since >= self.sort_every):
-                st.sort(fp)
+                st.sort(fp, self.refine_axis, self.refine_bits)
                 since = 0
             k = min(per, end - step)
             if sort:

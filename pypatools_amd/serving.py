@@ -28,7 +28,7 @@ import numpy as np
 from . import _lib
 from .batch import BATCH_ALGORITHMS, make_map
 from .device import DeviceArray, mem_info, synchronize
-from .engine import U_STATE, ParticleState
+from .engine import DEFAULT_REFINE_BITS, U_STATE, ParticleState, motion_axis
 from .interpolators import DeviceFieldMap
 
 
@@ -53,6 +53,7 @@ class Job:
         self.map_index = int(map_index)
         self.escale, self.bscale = float(escale), float(bscale)
         self.arrival = arrival
+        self.refine_axis = motion_axis(self.v0)
         self.id = Job._next_id
         Job._next_id += 1
         self.state = "new"
@@ -182,7 +183,8 @@ class TrackingServer:
 
     def _sort_job(self, job):
         ws = self._workspace(job.n)
-        _lib.call("pp_sort_state", self._range_args(job), self.pool.ids.view_ptr(job.offset), ws.ptr, ws.nbytes, None)
+        _lib.call("pp_sort_state_ex", self._range_args(job), None, None, 0, job.refine_axis, DEFAULT_REFINE_BITS,
+                  self.pool.ids.view_ptr(job.offset), ws.ptr, ws.nbytes, None)
 
     def _admit(self, job, offset):
         job.offset = offset
