@@ -123,7 +123,9 @@ struct GenericFE32 {
     }
 };
 
-template <int ALG, class FE>
+// PRE: the field evaluator returns h E and h B (h = q/m dt / 2) for the
+// Boris-family steps, folding the scale into its field multipliers.
+template <int ALG, bool PRE = false, class FE>
 DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64_t gs, const FE& fe,
                     const AlgConst& p) {
     const float inv_c2 = (float)(1.0 / p.c2);
@@ -159,10 +161,10 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
     } else {
         fe(i, r.x, r.y, r.z, gs, 0, E, B);
         if constexpr (ALG == PP_ALG_BORIS) {
-            const f3 dE = h * E;
+            const f3 dE = PRE ? E : h * E;
             const d3 vm = addf(v, dE);
             const f3 vmf = tof(vm);
-            const f3 t = h * B;
+            const f3 t = PRE ? B : h * B;
             const f3 s = (2.f * rcpf(1.f + norm2f(t))) * t;
             const f3 vp = vmf + crossf(vmf, t);
             const f3 dv = crossf(vp, s) + dE;
@@ -170,11 +172,11 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
             r = r + v * dt;
             vrec = v;
         } else if constexpr (ALG == PP_ALG_BORIS_REL) {
-            const f3 dE = h * E;
+            const f3 dE = PRE ? E : h * E;
             const d3 um = addf(v, dE);
             const f3 umf = tof(um);
             const float gm = 1.f + gamma_m1_from_u(umf, inv_c2);
-            const f3 t = (h * rcpf(gm)) * B;
+            const f3 t = (PRE ? rcpf(gm) : h * rcpf(gm)) * B;
             const f3 s = (2.f * rcpf(1.f + norm2f(t))) * t;
             const f3 up = umf + crossf(umf, t);
             const f3 du = crossf(up, s) + dE;
@@ -184,8 +186,8 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
         } else if constexpr (ALG == PP_ALG_VAY) {
             const f3 uf = tof(v);
             const float g = 1.f + gamma_m1_from_u(uf, inv_c2);
-            const f3 tau = h * B;
-            const f3 du1 = (2.f * h) * E + crossf(rcpf(g) * uf, tau);
+            const f3 tau = PRE ? B : h * B;
+            const f3 du1 = (PRE ? 2.f : 2.f * h) * E + crossf(rcpf(g) * uf, tau);
             const d3 up = addf(v, du1);
             const f3 upf = tof(up);
             const float tau2 = norm2f(tau);
@@ -312,13 +314,33 @@ DEV double2 ld2d(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return d;
 }
 
-DEV float ceval(float4 lo, float4 hi, float tx, float ty, float tz) {
-    const float p1 = fmaf(ty, hi.x, lo.y);
-    const float p2 = fmaf(tx, hi.w, hi.z);
-    const float p3 = fmaf(tx, hi.y, lo.w);
-    const float p4 = fmaf(ty, p2, p3);
-    const float r = fmaf(ty, lo.z, fmaf(tx, p1, lo.x));
-    return fmaf(tz, p4, r);
+// Trilinear monomials of the cell-local coordinates, shared by the 6 components.
+struct Mono {
+    float x, y, z, xy, xz, yz, xyz;
+};
+
+DEV Mono monomials(float tx, float ty, float tz) {
+    Mono w;
+    w.x = tx;
+    w.y = ty;
+    w.z = tz;
+    w.xy = tx * ty;
+    w.xz = tx * tz;
+    w.yz = ty * tz;
+    w.xyz = w.xy * tz;
+    return w;
+}
+
+// One component from its 8 coefficients (of 1, x, y, z, xy, xz, yz, xyz) as a
+// chain of 7 FMAs; the scalar-cache path below evaluates the same chain.
+DEV float ceval(float4 lo, float4 hi, const Mono& w) {
+    float r = fmaf(lo.y, w.x, lo.x);
+    r = fmaf(lo.z, w.y, r);
+    r = fmaf(lo.w, w.z, r);
+    r = fmaf(hi.x, w.xy, r);
+    r = fmaf(hi.y, w.xz, r);
+    r = fmaf(hi.z, w.yz, r);
+    return fmaf(hi.w, w.xyz, r);
 }
 
 DEV double cevald(double2 ab, double2 cd, double2 ef, double2 gh, double tx, double ty, double tz) {
@@ -339,16 +361,21 @@ DEV uint32_t hot_axis(double x, double id, double c, int nc, double& t, bool& ok
     return (uint32_t)fu;
 }
 
-DEV uint32_t hot_axis_periodic(double z, double id, double c, int nc, double inv_n, double& t) {
+// periodic axis: k = floor(u) mod nc in integer arithmetic.  floor(u) is clamped
+// to +-2^29 (NaN goes to -2^29) and offset by bias = nc * floor(2^30 / nc)
+// (>= 2^29 for nc <= 2^29) into [0, 2^31); q = mulhi(ku, floor((2^32-1)/nc)) is
+// the quotient or one less, so one conditional subtraction leaves k in [0, nc)
+// for every input, and k = floor(u) mod nc whenever |floor(u)| <= 2^29.
+DEV uint32_t hot_axis_periodic(double z, double id, double c, uint32_t nc, uint32_t magic, uint32_t bias,
+                               double& t) {
     const double u = fma(z, id, c);
     const double fu = floor(u);
     t = u - fu;
-    const double ncd = (double)nc;
-    double kd = fu - floor(fu * inv_n) * ncd;
-    if (kd >= ncd) kd -= ncd;
-    if (kd < 0.0) kd += ncd;
-    kd = fmin(fmax(kd, 0.0), ncd - 1.0);
-    return (uint32_t)kd;
+    const int32_t ki = (int32_t)fmin(fmax(fu, -536870912.0), 536870912.0);
+    const uint32_t ku = (uint32_t)ki + bias;
+    uint32_t k = ku - __umulhi(ku, magic) * nc;
+    if (k >= nc) k -= nc;
+    return k;
 }
 
 template <bool PERIODIC>
@@ -358,7 +385,7 @@ DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& t
     const uint32_t i = hot_axis(x, m.id0, m.c0, m.nc0, tx, ok);
     const uint32_t j = hot_axis(y, m.id1, m.c1, m.nc1, ty, ok);
     uint32_t k;
-    if constexpr (PERIODIC) k = hot_axis_periodic(z, m.id2, m.c2, m.nc2, m.inv_nz, tz);
+    if constexpr (PERIODIC) k = hot_axis_periodic(z, m.id2, m.c2, (uint32_t)m.nc2, m.zmagic, m.zbias, tz);
     else k = hot_axis(z, m.id2, m.c2, m.nc2, tz, ok);
     return (i * (uint32_t)m.nc1 + j) * (uint32_t)m.nc2 + k;
 }
@@ -368,25 +395,43 @@ DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& t
 // address become scalar loads (s_load_dwordx16) into SGPRs.
 typedef const __attribute__((address_space(4))) float* cfloat_p;
 
-DEV float ceval_s(cfloat_p q, float tx, float ty, float tz) {
-    const float p1 = fmaf(ty, q[4], q[1]);
-    const float p2 = fmaf(tx, q[7], q[6]);
-    const float p3 = fmaf(tx, q[5], q[3]);
-    const float p4 = fmaf(ty, p2, p3);
-    const float r = fmaf(ty, q[2], fmaf(tx, p1, q[0]));
-    return fmaf(tz, p4, r);
+DEV float ceval_s(cfloat_p q, const Mono& w) {
+    float r = fmaf(q[1], w.x, q[0]);
+    r = fmaf(q[2], w.y, r);
+    r = fmaf(q[3], w.z, r);
+    r = fmaf(q[4], w.xy, r);
+    r = fmaf(q[5], w.xz, r);
+    r = fmaf(q[6], w.yz, r);
+    return fmaf(q[7], w.xyz, r);
 }
 #endif
+
+// Field multipliers of the FP32 hot gather: E = kE * e and B = kB * b inside the
+// map, kE * fill and kB * fill outside.  kE, kB are the field scales, times
+// h = q/m dt / 2 for the Boris-family steps (step_mixed<ALG, true>).
+struct HotScale {
+    float kE, kB, kEf, kBf;
+};
+
+DEV HotScale hot_scale(const HotMap& m, double h) {
+    HotScale s;
+    s.kE = (float)(m.sE * h);
+    s.kB = (float)(m.sB * h);
+    s.kEf = s.kE * (float)m.fill;
+    s.kBf = s.kB * (float)m.fill;
+    return s;
+}
 
 template <bool PERIODIC>
 struct HotFE32 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
+    HotScale k;
     DEV void operator()(int64_t, double x, double y, double z, int64_t, int, f3& E, f3& B) const {
         double tx, ty, tz;
         bool ok;
         const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
-        const float fx = (float)tx, fy = (float)ty, fz = (float)tz;
+        const Mono w = monomials((float)tx, (float)ty, (float)tz);
         float v[6];
 #if defined(__HIP_DEVICE_COMPILE__)
         // A wavefront whose particles all sit in one cell (the common case after a
@@ -396,18 +441,19 @@ struct HotFE32 {
         if (__builtin_amdgcn_ballot_w64(cell != c0) == 0) {
             const cfloat_p q = (cfloat_p)m.coef + (size_t)c0 * 48u;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, fx, fy, fz);
+            for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, w);
         } else
 #endif
         {
             const uint32_t base = cell * (48u * 4u);
 #pragma unroll
             for (int c = 0; c < 6; ++c)
-                v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), fx, fy, fz);
+                v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
         }
-        const float fl = (float)m.fill, sE = (float)m.sE, sB = (float)m.sB;
-        E = mkf(sE * (ok ? v[0] : fl), sE * (ok ? v[1] : fl), sE * (ok ? v[2] : fl));
-        B = mkf(sB * (ok ? v[3] : fl), sB * (ok ? v[4] : fl), sB * (ok ? v[5] : fl));
+        const float me = ok ? k.kE : 0.f, ae = ok ? 0.f : k.kEf;
+        const float mb = ok ? k.kB : 0.f, ab = ok ? 0.f : k.kBf;
+        E = mkf(fmaf(v[0], me, ae), fmaf(v[1], me, ae), fmaf(v[2], me, ae));
+        B = mkf(fmaf(v[3], mb, ab), fmaf(v[4], mb, ab), fmaf(v[5], mb, ab));
     }
 };
 
@@ -445,6 +491,10 @@ DEV AlgConst hot_const(const HotParams& p) {
     return a;
 }
 
+// Boris-family steps take h E and h B straight from the FP32 hot gather
+template <int ALG>
+constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY;
+
 template <int ALG, bool MIXED, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -457,7 +507,8 @@ __global__ void __launch_bounds__(256) k_hot(const HotParams p) {
     const double dt = p.dt;
     if (p.kick_only) {
         if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
-            if constexpr (MIXED) v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC>{p.m, rs}, ac);
+            if constexpr (MIXED)
+                v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC>{p.m, rs, hot_scale(p.m, 1.0)}, ac);
             else v = kick64<ALG>(i, r, v, dt, p.step0, HotFE64<PERIODIC>{p.m, rs}, ac);
             p.vx[i] = v.x;
             p.vy[i] = v.y;
@@ -466,12 +517,14 @@ __global__ void __launch_bounds__(256) k_hot(const HotParams p) {
         return;
     }
     const float h = (float)(0.5 * p.qm * dt);
+    constexpr bool PRE = kPrescaled<ALG>;
+    const HotFE32<PERIODIC> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0)};
     const bool rec_on = p.r_hist != nullptr;
     int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
-        if constexpr (MIXED) step_mixed<ALG>(i, r, v, vrec, dt, h, p.step0 + s, HotFE32<PERIODIC>{p.m, rs}, ac);
+        if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC>{p.m, rs}, ac);
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
@@ -498,8 +551,10 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
                      __amdgpu_buffer_rsrc_t rs, const AlgConst& ac) {
     d3 vrec = v;
     const float h = (float)(0.5 * ac.qm * dt);
+    constexpr bool PRE = kPrescaled<ALG>;
+    const HotFE32<PERIODIC> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0)};
     for (int64_t s = 0; s < ns; ++s) {
-        if constexpr (MIXED) step_mixed<ALG>(i, r, v, vrec, dt, h, step0 + s, HotFE32<PERIODIC>{m, rs}, ac);
+        if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs}, ac);
     }
 }

@@ -394,6 +394,10 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
     h.c1 = -m.lo[1] * m.inv_d[1];
     h.c2 = -m.lo[2] * m.inv_d[2];
     h.inv_nz = m.inv_nz;
+    if (m.nc[2] > 0) {
+        h.zmagic = (uint32_t)(0xFFFFFFFFull / (uint64_t)m.nc[2]);
+        h.zbias = (uint32_t)m.nc[2] * ((1u << 30) / (uint32_t)m.nc[2]);
+    }
     h.sE = 1.0;
     h.sB = 1.0;
     h.fill = m.fill;

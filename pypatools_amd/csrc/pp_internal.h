@@ -74,6 +74,8 @@ struct HotMap {
     double inv_nz;          // 1 / nc2 (periodic z)
     double sE, sB;          // scale1 * scale2 of E and B
     double fill;            // value outside the grid (Field: 0)
+    uint32_t zmagic;        // floor((2^32 - 1) / nc2): quotient estimate for the periodic z wrap
+    uint32_t zbias;         // nc2 * floor(2^30 / nc2): makes the z cell index non-negative before the wrap
 };
 
 struct HotParams {
