@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--steps-per-launch", type=int, default=0)
-    ap.add_argument("--sort-every", type=int, default=250, help="steps between cell sorts (0: never)")
-    ap.add_argument("--refine-bits", type=int, default=3, help="sub-cell sort bins along the motion (2^bits)")
+    ap.add_argument("--sort-every", type=int, default=500, help="steps between cell sorts (0: never)")
+    ap.add_argument("--refine-bits", type=int, default=0, help="sub-cell sort bins along the motion (2^bits)")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: library default 256)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--bunches", type=int, default=512)
@@ -72,6 +72,30 @@ def copy_probe(mib):
     _lib.call("pp_aos_to_soa", src.ptr, n, dst[0].ptr, dst[1].ptr, dst[2].ptr, None)
     synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 24, "copy_probe_bytes_written": n * 24}), flush=True)
+
+
+def gather_probe(fe, fb, nthreads, iters):
+    """Measured peak of the gather path: the hot kernel's 12 x 16 B buffer loads per cell, no push math.
+
+    mode 0 = one cell per 64-lane wavefront (the vector memory path's return-data limit, the
+    ceiling for cell-sorted particles), mode 1 = a cell per lane from an L2-resident footprint.
+    Returns GB/s of gather bytes (192 B per cell read) for each, timed with HIP events.
+    """
+    from pypatools_amd import _lib
+    from pypatools_amd.batch import make_map
+    from pypatools_amd.device import DeviceArray, Event, synchronize
+    m = make_map(fe, fb)
+    out = DeviceArray((nthreads,), np.float32)
+    res = {}
+    for mode, ncell, name in ((0, 1024, "wave_uniform_cells_GBps"), (1, 4096, "per_lane_cells_L2_GBps")):
+        _lib.call("pp_gather_probe", m.handle, mode, ncell, iters, nthreads, out.ptr, None)
+        e0, e1 = Event(), Event()
+        e0.record()
+        _lib.call("pp_gather_probe", m.handle, mode, ncell, iters, nthreads, out.ptr, None)
+        e1.record()
+        synchronize()
+        res[name] = nthreads * iters * 192 / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    return res
 
 
 def cpu_baseline(w, seconds=15.0):
@@ -388,6 +412,8 @@ def main():
     # roofline of the dominant kernel (k_hot<boris>): algorithmic gather bytes per launch / launch time
     units = M * steps_per_launch
     achieved = GATHER_BYTES[args.precision] * units / (avg_ms * 1e-3) / 1e9
+    probe = gather_probe(fe, fb, M, steps_per_launch)
+    gpeak = probe["wave_uniform_cells_GBps"]
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
@@ -406,7 +432,10 @@ def main():
                      "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true> (boris, "
                                f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.2f} ms)",
                      "algorithmic_bytes_per_particle_step": GATHER_BYTES[args.precision],
-                     "gflops": FLOPS_PER_STEP * units / (avg_ms * 1e-3) / 1e9},
+                     "gflops": FLOPS_PER_STEP * units / (avg_ms * 1e-3) / 1e9,
+                     "gather_roofline": {"level": "vector memory path (buffer loads through L1), measured peak",
+                                         "peak": gpeak, "unit": "GB/s", "frac": achieved / gpeak,
+                                         "probe": probe}},
         "extra": extra,
     }
     out["config"]["sort_every"] = args.sort_every

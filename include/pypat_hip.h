@@ -127,7 +127,7 @@ typedef struct pp_push_args {
     int64_t n_records;
     int32_t kick_only;       /* 1: velocity-only kick with this dt (Boris staggering,
                                 pusher.py:946-958, 992-1007); nsteps must be 1 */
-    int32_t block_size;      /* 0 = library default */
+    int32_t block_size;      /* threads per block: 64, 128, 192 or 256; 0 = library default (256) */
 } pp_push_args;
 
 /* ---- runtime ------------------------------------------------------------ */
@@ -296,6 +296,16 @@ int pp_aos_to_soa(const double* aos, int64_t m, double* a, double* b, double* c,
 int pp_soa_to_aos(const double* a, const double* b, const double* c, int64_t m, double* aos, pp_stream s);
 int pp_v_to_u(double* vx, double* vy, double* vz, int64_t m, pp_stream s);   /* u = gamma(v) v, pusher.py:74-96 clamp */
 int pp_u_to_v(double* ux, double* uy, double* uz, int64_t m, pp_stream s);   /* v = u / sqrt(1 + u^2/c^2) */
+
+/* Gather-bandwidth probe (measurement utility; no counterpart in the reference).
+   nthreads threads each read `iters` cells of 192 B from the FP32 coefficient table
+   of `map` through the same buffer-load path as the FP32 hot kernels and fold them
+   into out[thread].  mode 0: the cell is uniform per 64-lane wavefront; mode 1: each
+   lane reads its own cell.  Cells follow a hashed sequence over the first `ncell`
+   cells (a power of two, at most the map's cell count), so the footprint sets the
+   cache level that serves the reads.  Bytes read = nthreads * iters * 192. */
+int pp_gather_probe(const pp_map* map, int32_t mode, uint32_t ncell, int32_t iters, int64_t nthreads, float* out,
+                    pp_stream s);
 
 #ifdef __cplusplus
 }

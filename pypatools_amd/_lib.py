@@ -24,6 +24,7 @@ MAP_PERIODIC_Z, MAP_NO_FP32, MAP_NO_COEF = 1, 2, 4
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int64 = ctypes.c_int64
 c_int32 = ctypes.c_int32
+c_uint32 = ctypes.c_uint32
 c_void_p = ctypes.c_void_p
 
 
@@ -145,6 +146,7 @@ SIGNATURES = {
     "pp_graph_destroy": (c_int32, [c_void_p]),
     "pp_add_i64": (c_int32, [c_void_p, c_int64, c_void_p]),
     "pp_aos_to_soa": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pp_gather_probe": (c_int32, [c_void_p, c_int32, c_uint32, c_int32, c_int64, c_void_p, c_void_p]),
     "pp_soa_to_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "pp_v_to_u": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "pp_u_to_v": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),

@@ -422,6 +422,12 @@ DEV HotScale hot_scale(const HotMap& m, double h) {
     return s;
 }
 
+// LDS staging of a wavefront's cells: up to kStageSlots distinct cells, 12 x 16 B
+// each, padded to 13 x 16 B so that slots fall in different LDS banks.
+constexpr int kStageSlots = 10;
+constexpr int kStageWaves = 4;      // wavefronts per block (the hot kernels launch <= 256 threads)
+constexpr int kStagePitch = 13;     // float4 per slot
+
 template <bool PERIODIC>
 struct HotFE32 {
     const HotMap& m;
@@ -433,18 +439,55 @@ struct HotFE32 {
         const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
         const Mono w = monomials((float)tx, (float)ty, (float)tz);
         float v[6];
+        bool done = false;
 #if defined(__HIP_DEVICE_COMPILE__)
-        // A wavefront whose particles all sit in one cell (the common case after a
-        // cell sort) reads the cell's 192 B once through the scalar cache; other
-        // wavefronts gather per lane through the vector memory path.
+        // Three ways to the cell coefficients, chosen per wavefront:
+        //  - all lanes in one cell (common right after a cell sort): the cell's
+        //    192 B are read once through the scalar cache;
+        //  - a full wavefront over at most kStageSlots distinct cells: each
+        //    distinct cell is loaded once (12 lanes x 16 B) into the wavefront's
+        //    LDS slots and every lane reads its cell from LDS;
+        //  - otherwise every lane gathers its cell through the vector memory path.
+        // All three evaluate the same FMA chain on the same values.
         const uint32_t c0 = __builtin_amdgcn_readfirstlane(cell);
         if (__builtin_amdgcn_ballot_w64(cell != c0) == 0) {
             const cfloat_p q = (cfloat_p)m.coef + (size_t)c0 * 48u;
 #pragma unroll
             for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, w);
-        } else
+            done = true;
+        } else if (__builtin_amdgcn_ballot_w64(true) == ~0ull) {
+            __shared__ float4 stage[kStageWaves][kStageSlots * kStagePitch];
+            const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+            const uint32_t lslot = lane / 12u, lpart = lane - 12u * lslot;   // staging role of this lane
+            uint64_t todo = ~0ull;
+            uint32_t myslot = 0, ld0 = 0, ld1 = 0;
+            int ns = 0;
+            while (todo != 0 && ns < kStageSlots) {
+                const uint32_t c = __builtin_amdgcn_readlane(cell, (int)__builtin_ctzll(todo));
+                const bool mine = cell == c;
+                todo &= ~__builtin_amdgcn_ballot_w64(mine);
+                myslot = mine ? (uint32_t)ns : myslot;
+                if (ns < 5) ld0 = lslot == (uint32_t)ns ? c : ld0;
+                else ld1 = lslot == (uint32_t)(ns - 5) ? c : ld1;
+                ++ns;
+            }
+            if (todo == 0) {
+                float4* wv = stage[threadIdx.x >> 6];
+                if (lslot < 5u && lslot < (uint32_t)ns) wv[lslot * kStagePitch + lpart] = ld4f(rs, ld0 * 192u + lpart * 16u);
+                if (lslot < 5u && lslot + 5u < (uint32_t)ns)
+                    wv[(lslot + 5u) * kStagePitch + lpart] = ld4f(rs, ld1 * 192u + lpart * 16u);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const float4* q = wv + myslot * kStagePitch;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
+                __builtin_amdgcn_wave_barrier();
+                done = true;
+            }
+        }
 #endif
-        {
+        if (!done) {
             const uint32_t base = cell * (48u * 4u);
 #pragma unroll
             for (int c = 0; c < 6; ++c)
@@ -495,8 +538,10 @@ DEV AlgConst hot_const(const HotParams& p) {
 template <int ALG>
 constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY;
 
+// 4 wavefronts per SIMD (<= 128 VGPRs) keeps enough loads and LDS reads in flight;
+// the compiler would otherwise spend registers for a deeper load pipeline at 3.
 template <int ALG, bool MIXED, bool PERIODIC>
-__global__ void __launch_bounds__(256) k_hot(const HotParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     const __amdgpu_buffer_rsrc_t rs =
@@ -560,7 +605,7 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
 }
 
 template <bool MIXED, bool PERIODIC>
-__global__ void __launch_bounds__(256) k_batched(const BatchParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_batched(const BatchParams p) {
     const int64_t t = blockIdx.x;
     if (t >= p.ntiles) return;
     const int32_t b = p.tile_bunch[t];
@@ -615,7 +660,40 @@ __global__ void __launch_bounds__(256) k_batched(const BatchParams p) {
     p.vz[i] = v.z;
 }
 
+// ---------------------------------------------------------------------------
+// Gather-bandwidth probe: the hot kernels' 12 x 16 B buffer loads per cell with
+// no push arithmetic, to measure the rate the vector memory path sustains for
+// wave-uniform (mode 0) and per-lane (mode 1) cells.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_gather_probe(const float* table, uint32_t bytes, uint32_t mask, int mode,
+                                                      int iters, int64_t nthreads, float* __restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nthreads) return;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(table), (short)0, (int)bytes, kBufferDword3);
+    uint32_t h = (uint32_t)(mode == 0 ? (gid >> 6) : gid) * 2654435761u + 12345u;
+    float acc = 0.f;
+    for (int it = 0; it < iters; ++it) {
+        const uint32_t base = ((h >> 7) & mask) * 192u;
+        h = h * 1664525u + 1013904223u;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) {
+            const float4 q = ld4f(rs, base + 16u * c);
+            acc += (q.x + q.y) + (q.z + q.w);
+        }
+    }
+    out[gid] = acc;
+}
+
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+hipError_t launch_gather_probe(const float* table, uint32_t bytes, uint32_t ncell, int mode, int iters,
+                               int64_t nthreads, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_probe, dim3(grid_for(nthreads, 256)), dim3(256), 0, s, table, bytes, ncell - 1,
+                       mode, iters, nthreads, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int block, hipStream_t s) {
     const dim3 g((unsigned)p.ntiles), b(block);

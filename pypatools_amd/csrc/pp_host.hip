@@ -483,8 +483,8 @@ int pp_push(const pp_push_args* a, pp_stream s) {
         if (!a->r_hist || !a->v_hist) return fail(PP_ERR_INVALID, "r_hist and v_hist go together");
         if (a->rec_every < 1 || a->n_records < 1) return fail(PP_ERR_INVALID, "rec_every and n_records must be >= 1");
     }
-    if (a->block_size < 0 || a->block_size > 1024 || (a->block_size % 64) != 0)
-        return fail(PP_ERR_INVALID, "block_size must be a multiple of 64 in [64, 1024] (or 0)");
+    if (a->block_size < 0 || a->block_size > 256 || (a->block_size % 64) != 0)
+        return fail(PP_ERR_INVALID, "block_size must be 64, 128, 192 or 256 (or 0)");
     PushParams p;
     memset(&p, 0, sizeof(p));
     if ((rc = make_field(a->efield, a->precision, p.E))) return rc;
@@ -850,6 +850,29 @@ static int util_common(int which, double* a, double* b, double* c, double* aos, 
         return fail(PP_ERR_INVALID, "null device pointer");
     hipError_t e = pp_strict::launch_util(which, a, b, c, aos, m, kClight * kClight, stream_of(s));
     if (e != hipSuccess) return hip_fail(e, "conversion kernel launch");
+    return PP_OK;
+}
+
+int pp_gather_probe(const pp_map* map, int32_t mode, uint32_t ncell, int32_t iters, int64_t nthreads, float* out,
+                    pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!map || !out) return fail(PP_ERR_INVALID, "null argument");
+    if (map->device != g_device) return fail(PP_ERR_INVALID, "map lives on another device");
+    const MapDesc& m = map->d;
+    if (!m.coef32 || m.ncomp != 6) return fail(PP_ERR_UNSUPPORTED, "probe needs a 6-component map with an FP32 table");
+    const uint64_t cells = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2];
+    const uint64_t bytes = cells * 192u;
+    if (bytes >= 0x7FFFFFFFull) return fail(PP_ERR_UNSUPPORTED, "map too large for 32-bit offsets");
+    if (mode != 0 && mode != 1) return fail(PP_ERR_INVALID, "mode must be 0 or 1");
+    if (ncell == 0 || (ncell & (ncell - 1)) || ncell > cells)
+        return fail(PP_ERR_INVALID, "ncell must be a power of two no larger than the map's %llu cells",
+                    (unsigned long long)cells);
+    if (iters < 0 || nthreads < 0 || nthreads > ((int64_t)1 << 31)) return fail(PP_ERR_INVALID, "bad sizes");
+    if (nthreads == 0) return PP_OK;
+    hipError_t e = pp_fast::launch_gather_probe(m.coef32, (uint32_t)bytes, ncell, mode, iters, nthreads, out,
+                                                stream_of(s));
+    if (e != hipSuccess) return hip_fail(e, "gather probe launch");
     return PP_OK;
 }
 

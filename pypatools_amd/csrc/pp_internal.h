@@ -144,6 +144,8 @@ hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int 
 hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int block, hipStream_t s);
 hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int64_t m, double rf, double* out,
                              hipStream_t s);
+hipError_t launch_gather_probe(const float* table, uint32_t bytes, uint32_t ncell, int mode, int iters,
+                               int64_t nthreads, float* out, hipStream_t s);
 }  // namespace pp_fast
 
 namespace pp_sort {

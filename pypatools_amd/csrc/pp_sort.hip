@@ -54,22 +54,39 @@ __global__ void __launch_bounds__(256) k_keys(const MapDesc m, const double* __r
     slots[i] = (uint32_t)i;
 }
 
-__global__ void __launch_bounds__(256) k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ perm,
-                                                    int64_t n, double* __restrict__ dst) {
+// The six state arrays and the id array, permuted together in one pass.
+struct StateArrays {
+    double* a[6];
+    int32_t* ids;
+};
+
+__global__ void __launch_bounds__(256) k_gather_state(const StateArrays src, const uint32_t* __restrict__ perm,
+                                                      int64_t n, const StateArrays dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
+    if (i >= n) return;
+    const uint32_t p = perm[i];
+    if (p >= (uint64_t)n) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst.a[k][i] = src.a[k][p];
+    dst.ids[i] = src.ids[p];
 }
 
-__global__ void __launch_bounds__(256) k_gather_i32(const int32_t* __restrict__ src, const uint32_t* __restrict__ perm,
-                                                    int64_t n, int32_t* __restrict__ dst) {
+__global__ void __launch_bounds__(256) k_scatter_state(const StateArrays src, int64_t n, const StateArrays dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
+    if (i >= n) return;
+    const int32_t t = src.ids[i];
+    if ((uint32_t)t >= (uint64_t)n) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst.a[k][t] = src.a[k][i];
 }
 
-__global__ void __launch_bounds__(256) k_scatter_f64(const double* __restrict__ src, const int32_t* __restrict__ ids,
-                                                     int64_t n, double* __restrict__ dst) {
+__global__ void __launch_bounds__(256) k_copy_state(const StateArrays src, int64_t n, const StateArrays dst,
+                                                    int with_ids) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (uint32_t)ids[i] < (uint64_t)n) dst[ids[i]] = src[i];
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst.a[k][i] = src.a[k][i];
+    dst.ids[i] = with_ids ? src.ids[i] : (int32_t)i;
 }
 
 __global__ void __launch_bounds__(256) k_iota(int32_t* ids, int64_t n) {
@@ -85,14 +102,31 @@ static inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256);
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// workspace: keys_in, keys_out, slots_in, slots_out (4n each), tmp (8n), cub temp
+// workspace: keys_in, keys_out, slots_in, slots_out (4n each), a staging copy of
+// the state (6 x 8n + 4n), cub temp
+static size_t staging_bytes(int64_t n) { return 6 * align256(8 * (size_t)n) + align256(4 * (size_t)n); }
+
 hipError_t workspace_bytes(int64_t n, size_t* bytes) {
     size_t cub_bytes = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                                       (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
     if (e != hipSuccess) return e;
-    *bytes = 4 * align256(4 * (size_t)n) + align256(8 * (size_t)n) + align256(cub_bytes);
+    *bytes = 4 * align256(4 * (size_t)n) + staging_bytes(n) + align256(cub_bytes);
     return hipSuccess;
+}
+
+static pp_sortk::StateArrays staging(char* base, int64_t n) {
+    pp_sortk::StateArrays t;
+    for (int k = 0; k < 6; ++k) t.a[k] = reinterpret_cast<double*>(base + k * align256(8 * (size_t)n));
+    t.ids = reinterpret_cast<int32_t*>(base + 6 * align256(8 * (size_t)n));
+    return t;
+}
+
+static pp_sortk::StateArrays state_of(double* arrs[6], int32_t* ids) {
+    pp_sortk::StateArrays t;
+    for (int k = 0; k < 6; ++k) t.a[k] = arrs[k];
+    t.ids = ids;
+    return t;
 }
 
 static int key_bits(const MapDesc& m) {
@@ -114,24 +148,18 @@ hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n
     uint32_t* keys_out = reinterpret_cast<uint32_t*>(w + q);
     uint32_t* slots_in = reinterpret_cast<uint32_t*>(w + 2 * q);
     uint32_t* perm = reinterpret_cast<uint32_t*>(w + 3 * q);
-    double* tmp = reinterpret_cast<double*>(w + 4 * q);
-    char* cub_tmp = w + 4 * q + align256(8 * (size_t)n);
-    size_t cub_bytes = ws_bytes - (4 * q + align256(8 * (size_t)n));
+    const pp_sortk::StateArrays tmp = staging(w + 4 * q, n);
+    char* cub_tmp = w + 4 * q + staging_bytes(n);
+    size_t cub_bytes = ws_bytes - (4 * q + staging_bytes(n));
     hipLaunchKernelGGL(pp_sortk::k_keys, dim3(grid_for(n)), dim3(256), 0, s, m, arrs[0], arrs[1], arrs[2], n, group,
                        group_shift, refine_axis, refine_bits, keys_in, slots_in);
     const int bits = group ? 32 : (key_bits(m) + refine_bits > 32 ? 32 : key_bits(m) + refine_bits);
     e = hipcub::DeviceRadixSort::SortPairs(cub_tmp, cub_bytes, keys_in, keys_out, slots_in, perm, (int)n, 0, bits,
                                            s);
     if (e != hipSuccess) return e;
-    for (int a = 0; a < 6; ++a) {
-        hipLaunchKernelGGL(pp_sortk::k_gather_f64, dim3(grid_for(n)), dim3(256), 0, s, arrs[a], perm, n, tmp);
-        e = hipMemcpyAsync(arrs[a], tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return e;
-    }
-    int32_t* itmp = reinterpret_cast<int32_t*>(tmp);
-    hipLaunchKernelGGL(pp_sortk::k_gather_i32, dim3(grid_for(n)), dim3(256), 0, s, ids, perm, n, itmp);
-    e = hipMemcpyAsync(ids, itmp, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
+    const pp_sortk::StateArrays st = state_of(arrs, ids);
+    hipLaunchKernelGGL(pp_sortk::k_gather_state, dim3(grid_for(n)), dim3(256), 0, s, st, perm, n, tmp);
+    hipLaunchKernelGGL(pp_sortk::k_copy_state, dim3(grid_for(n)), dim3(256), 0, s, tmp, n, st, 1);
     return hipGetLastError();
 }
 
@@ -140,13 +168,10 @@ hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size
     hipError_t e = workspace_bytes(n, &need);
     if (e != hipSuccess) return e;
     if (ws_bytes < need) return hipErrorInvalidValue;
-    double* tmp = reinterpret_cast<double*>(static_cast<char*>(ws) + 4 * align256(4 * (size_t)n));
-    for (int a = 0; a < 6; ++a) {
-        hipLaunchKernelGGL(pp_sortk::k_scatter_f64, dim3(grid_for(n)), dim3(256), 0, s, arrs[a], ids, n, tmp);
-        e = hipMemcpyAsync(arrs[a], tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(pp_sortk::k_iota, dim3(grid_for(n)), dim3(256), 0, s, ids, n);
+    const pp_sortk::StateArrays tmp = staging(static_cast<char*>(ws) + 4 * align256(4 * (size_t)n), n);
+    const pp_sortk::StateArrays st = state_of(arrs, ids);
+    hipLaunchKernelGGL(pp_sortk::k_scatter_state, dim3(grid_for(n)), dim3(256), 0, s, st, n, tmp);
+    hipLaunchKernelGGL(pp_sortk::k_copy_state, dim3(grid_for(n)), dim3(256), 0, s, tmp, n, st, 0);
     return hipGetLastError();
 }
 

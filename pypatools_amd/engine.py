@@ -32,10 +32,10 @@ RK4_FAMILY = ("rk4", "rk4_rel")
 # particle-steps per fused launch (keeps single launches around a second or less)
 DEFAULT_LAUNCH_WORK = 2 ** 30
 # steps between cell sorts of the state when a field is a gridded map
-DEFAULT_SORT_EVERY = 250
+DEFAULT_SORT_EVERY = 500
 SORT_MIN_PARTICLES = 4096
 # sub-cell bins (2^bits) along the direction of motion in the sort key
-DEFAULT_REFINE_BITS = 3
+DEFAULT_REFINE_BITS = 0
 
 
 def motion_axis(v):
