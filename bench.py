@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 GATHER_BYTES = {"fp32_mixed": 192, "fp64_fast": 384, "fp64": 384}   # 8 corners x 6 comps x sizeof(map element)
 FLOPS_PER_STEP = 55 + 135                                          # non-rel Boris push + one 6-comp gather
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4        # wave64 VALU instructions per second, whole chip
 
 
 def parse():
@@ -414,11 +415,23 @@ def main():
     achieved = GATHER_BYTES[args.precision] * units / (avg_ms * 1e-3) / 1e9
     probe = gather_probe(fe, fb, M, steps_per_launch)
     gpeak = probe["wave_uniform_cells_GBps"]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get(args.precision)
+    # counters of the same kernel and workload from the committed rocprofv3 runs: HBM bytes per
+    # launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and VALU instructions per wavefront-step
+    traffic, valu = None, None
+    cfile = os.path.join(ROOT, "profiles", "pmc_counters.json")
+    if os.path.exists(cfile):
+        with open(cfile) as f:
+            rec = json.load(f).get(args.precision)
+        if rec and rec["steps_per_launch"] == steps_per_launch:
+            traffic = rec["hbm_bytes_per_launch"]
+        if rec:
+            wave_steps = -(-M // 64) * steps_per_launch
+            issue = VALU_ISSUE_PER_S * avg_ms * 1e-3                # wave64 VALU issue slots in one launch
+            valu = {"kind": "valu", "insts_per_wavefront_step": rec["valu_insts_per_wavefront_step"],
+                    "frac": rec["valu_insts_per_wavefront_step"] * wave_steps / issue,
+                    "peak": "256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz "
+                            "(78.6 TFLOP/s FP32 non-packed = FP64 vector peak)",
+                    "note": "instructions per wavefront-step from profiles/pmc_counters.json, launch time live"}
     out = {
         "metric": "particle-steps/sec on 1 MI355X, FP32-mixed vs FP64 at stated error; p99 job latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -433,7 +446,9 @@ def main():
                                f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.2f} ms)",
                      "algorithmic_bytes_per_particle_step": GATHER_BYTES[args.precision],
                      "gflops": FLOPS_PER_STEP * units / (avg_ms * 1e-3) / 1e9,
-                     "gather_roofline": {"level": "vector memory path (buffer loads through L1), measured peak",
+                     "limiting_roofline": valu,
+                     "gather_roofline": {"level": "vector memory path (buffer loads through L1), measured peak; "
+                                                  "the kernel stages cells through LDS and exceeds it",
                                          "peak": gpeak, "unit": "GB/s", "frac": achieved / gpeak,
                                          "probe": probe}},
         "extra": extra,
