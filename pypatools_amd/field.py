@@ -64,6 +64,10 @@ class FieldBase(ABC):
         return evaluate_on_device(self, pts_dev, m, precision, t)
 
 
+_FIELD_KWARGS = ("label", "dim", "field", "scaling", "units", "debug", "method", "interpolator_backend",
+                 "periodic_z", "precision")
+
+
 class Field(FieldBase):
     """0-D (constant), 1-D (z), 2-D (x, y) or 3-D gridded field."""
 
@@ -116,13 +120,44 @@ class Field(FieldBase):
     # ---- constructors ---------------------------------------------------------
     @classmethod
     def from_file(cls, filename, **kwargs):
+        """field.py:284-343: load a map file, choosing the loader by extension.
+
+        Keyword arguments that Field() takes (label, scaling, method, ...) go to the
+        Field; the rest go to the loader (e.g. ``r_cutoff`` for .dat, ``mirror`` for
+        .map).  Component arrays are reshaped onto the axes that have more than one
+        node, so 2-D maps stored as (nx, ny, 1) load as 2-D Fields.  Pickled fields
+        are not loaded: unpickling runs code from the file.
+        """
+        from .field_loaders import LOADERS
         if not os.path.exists(filename):
             raise FileNotFoundError(f"Field file not found: {filename}")
         _, ext = os.path.splitext(filename)
-        if ext not in (".pickle", ".h5", ".h5part", ".table", ".comsol", ".dat", ".map"):
+        if ext == ".pickle":
+            raise NotImplementedError("pickled fields are not loaded (unpickling executes code from the file); "
+                                      "save grid and values with numpy and use Field.from_arrays")
+        if ext not in LOADERS:
             raise ValueError(f"Unknown file extension: {ext}")
-        raise NotImplementedError(f"field file loaders ({ext}) are not part of the device engine; "
-                                  "load the grid and values and use Field.from_arrays")
+        field_kw = {k: kwargs.pop(k) for k in list(kwargs) if k in _FIELD_KWARGS}
+        data = LOADERS[ext](filename, **kwargs)
+        if data.get("coordinate_system") == "cylindrical":
+            raise ValueError("Field maps are Cartesian; load with cartesian_grid=True")
+        field = cls(**field_kw)
+        field._filename = filename
+        pts = [np.asarray(data["grid"][k], dtype=np.float64) for k in ("x", "y", "z")
+               if k in data["grid"] and len(data["grid"][k]) > 1]
+        field._dim = data["dim"]
+        field._metadata = data["metadata"]
+        shape = tuple(len(p) for p in pts)
+        for comp in ("x", "y", "z"):
+            if comp in data["values"]:
+                vals = np.asarray(data["values"][comp], dtype=np.float64)
+                if vals.shape != shape and vals.size == int(np.prod(shape)):
+                    vals = vals.reshape(shape)
+                interp = get_interpolator(points=tuple(pts), values=vals, bounds_error=False, fill_value=0.0,
+                                          method=field._method, backend=field._interpolator_backend)
+                interp.periodic_z = field._periodic_z
+                field._field[comp] = interp
+        return field
 
     @classmethod
     def from_arrays(cls, grid, values, **kwargs):
