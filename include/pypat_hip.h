@@ -297,6 +297,28 @@ int pp_soa_to_aos(const double* a, const double* b, const double* c, int64_t m, 
 int pp_v_to_u(double* vx, double* vy, double* vz, int64_t m, pp_stream s);   /* u = gamma(v) v, pusher.py:74-96 clamp */
 int pp_u_to_v(double* ux, double* uy, double* uz, int64_t m, pp_stream s);   /* v = u / sqrt(1 + u^2/c^2) */
 
+/* Beam moments of a device-resident state (ParticleDistribution, particles.py:693-843).
+   state: n, x..vz and optionally alive (other fields unused); u_state = 1 when the
+   velocity slots hold u = gamma v.  Momenta are beta*gamma = gamma v / c (physical,
+   not the reference's per-component conversion, particles.py:72-77); angles are
+   v / v_z with |v_z| < 1e-10 replaced by 1e-10 (particles.py:463-467); kinetic energy
+   is m (gamma - 1) from |p| (or p_z when z_energy), mass in MeV.
+   sums_dev[PP_MOMENT_SUMS]: n, sum x,y,z, sum px,py,pz, sum |p|, sum E, sum x',y',z',
+   min x,y,z, max x,y,z.  central_dev[PP_MOMENT_CENTRAL]: sums of (x-<x>)^2 (3),
+   (x'-<x'>)^2 (3), then (x-<x>)(x'-<x'>), (y-<y>)(y'-<y'>), (x-<x>)(y'-<y'>),
+   (y-<y>)(x'-<x'>), (E-<E>)^2.  Stream-ordered; workspace from
+   pp_beam_workspace_bytes. */
+#define PP_MOMENT_SUMS 18
+#define PP_MOMENT_CENTRAL 11
+int pp_beam_workspace_bytes(size_t* bytes);
+int pp_beam_moments(const pp_push_args* state, int32_t u_state, double mass_mev, int32_t z_energy, double* sums_dev,
+                    double* central_dev, void* workspace, size_t ws_bytes, pp_stream s);
+/* Particles inside the 4-rms ellipses gamma x^2 + 2 alpha x x' + beta x'^2 < 4 eps, for x and y
+   (particles.py:821-835).  twiss_dev[8] = alpha_x, beta_x, gamma_x, alpha_y, beta_y, gamma_y,
+   eps_x, eps_y; counts_dev[2]. */
+int pp_beam_ellipse_counts(const pp_push_args* state, int32_t u_state, const double* twiss_dev, double* counts_dev,
+                           void* workspace, size_t ws_bytes, pp_stream s);
+
 /* Gather-bandwidth probe (measurement utility; no counterpart in the reference).
    nthreads threads each read `iters` cells of 192 B from the FP32 coefficient table
    of `map` through the same buffer-load path as the FP32 hot kernels and fold them

@@ -61,6 +61,7 @@ class PPBunch(ctypes.Structure):
 
 
 MAX_BATCH_MAPS = 4
+MOMENT_SUMS, MOMENT_CENTRAL = 18, 11          # include/pypat_hip.h PP_MOMENT_SUMS, PP_MOMENT_CENTRAL
 
 
 class PPBatchArgs(ctypes.Structure):
@@ -147,6 +148,11 @@ SIGNATURES = {
     "pp_add_i64": (c_int32, [c_void_p, c_int64, c_void_p]),
     "pp_aos_to_soa": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_gather_probe": (c_int32, [c_void_p, c_int32, c_uint32, c_int32, c_int64, c_void_p, c_void_p]),
+    "pp_beam_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_size_t)]),
+    "pp_beam_moments": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, ctypes.c_double, c_int32, c_void_p,
+                                  c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "pp_beam_ellipse_counts": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, c_void_p, c_void_p, c_void_p,
+                                         ctypes.c_size_t, c_void_p]),
     "pp_soa_to_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "pp_v_to_u": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "pp_u_to_v": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),

@@ -853,6 +853,55 @@ static int util_common(int which, double* a, double* b, double* c, double* aos, 
     return PP_OK;
 }
 
+static int moment_args(const pp_push_args* a, int32_t u_state, double mass_mev, int32_t z_energy, MomentArgs& m) {
+    if (!a) return fail(PP_ERR_INVALID, "null state");
+    if (a->n < 0) return fail(PP_ERR_INVALID, "negative particle count");
+    if (a->n > 0 && (!a->x || !a->y || !a->z || !a->vx || !a->vy || !a->vz))
+        return fail(PP_ERR_INVALID, "null device pointer");
+    if (!(mass_mev > 0.0)) return fail(PP_ERR_INVALID, "mass_mev must be positive");
+    memset(&m, 0, sizeof(m));
+    m.x = a->x; m.y = a->y; m.z = a->z;
+    m.vx = a->vx; m.vy = a->vy; m.vz = a->vz;
+    m.alive = a->alive;
+    m.n = a->n;
+    m.u_state = u_state != 0;
+    m.z_energy = z_energy != 0;
+    m.mass_mev = mass_mev;
+    m.clight = kClight;
+    m.beta2_max = 0.9999;
+    return PP_OK;
+}
+
+int pp_beam_workspace_bytes(size_t* bytes) {
+    if (!bytes) return fail(PP_ERR_INVALID, "null argument");
+    *bytes = pp_beam::workspace_bytes();
+    return PP_OK;
+}
+
+int pp_beam_moments(const pp_push_args* state, int32_t u_state, double mass_mev, int32_t z_energy, double* sums_dev,
+                    double* central_dev, void* workspace, size_t ws_bytes, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    MomentArgs m;
+    if ((rc = moment_args(state, u_state, mass_mev, z_energy, m))) return rc;
+    if (!sums_dev || !central_dev || !workspace) return fail(PP_ERR_INVALID, "null output or workspace");
+    if (ws_bytes < pp_beam::workspace_bytes()) return fail(PP_ERR_INVALID, "workspace too small");
+    HIP_TRY(pp_beam::moments(m, sums_dev, central_dev, workspace, stream_of(s)), "beam moment kernels");
+    return PP_OK;
+}
+
+int pp_beam_ellipse_counts(const pp_push_args* state, int32_t u_state, const double* twiss_dev, double* counts_dev,
+                           void* workspace, size_t ws_bytes, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    MomentArgs m;
+    if ((rc = moment_args(state, u_state, 1.0, 0, m))) return rc;
+    if (!twiss_dev || !counts_dev || !workspace) return fail(PP_ERR_INVALID, "null argument");
+    if (ws_bytes < pp_beam::workspace_bytes()) return fail(PP_ERR_INVALID, "workspace too small");
+    HIP_TRY(pp_beam::ellipse_counts(m, twiss_dev, counts_dev, workspace, stream_of(s)), "ellipse count kernels");
+    return PP_OK;
+}
+
 int pp_gather_probe(const pp_map* map, int32_t mode, uint32_t ncell, int32_t iters, int64_t nthreads, float* out,
                     pp_stream s) {
     int rc = ensure_init();

@@ -148,6 +148,25 @@ hipError_t launch_gather_probe(const float* table, uint32_t bytes, uint32_t ncel
                                int64_t nthreads, float* out, hipStream_t s);
 }  // namespace pp_fast
 
+// SoA state viewed by the beam-moment reductions
+struct MomentArgs {
+    const double* x; const double* y; const double* z;
+    const double* vx; const double* vy; const double* vz;
+    const uint8_t* alive;       // NULL: every slot counts
+    int64_t n;
+    int32_t u_state;            // velocity slots hold u = gamma v
+    int32_t z_energy;           // kinetic energy from p_z only (Z_ENERGY setting)
+    double mass_mev, clight, beta2_max;
+};
+
+constexpr int kMomentBlocks = 1024;   // partials per reduction pass
+
+namespace pp_beam {
+size_t workspace_bytes();
+hipError_t moments(const MomentArgs& a, double* sums, double* central, void* ws, hipStream_t s);
+hipError_t ellipse_counts(const MomentArgs& a, const double* twiss_dev, double* counts, void* ws, hipStream_t s);
+}  // namespace pp_beam
+
 namespace pp_sort {
 hipError_t workspace_bytes(int64_t n, size_t* bytes);
 hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,

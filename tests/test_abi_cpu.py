@@ -49,3 +49,10 @@ def test_argument_validation_happens_before_any_device_call():
     assert b"constants" in lib.pp_last_error()
     a, b = ctypes.c_int64(), ctypes.c_int64()
     assert lib.pp_struct_sizes(None, ctypes.byref(b)) == _lib.PP_ERR_INVALID
+
+
+def test_header_defines_match_binding():
+    text = open(os.path.join(ROOT, "include", "pypat_hip.h")).read()
+    defs = dict(re.findall(r"#define\s+(PP_[A-Z_]+)\s+(\d+)", text))
+    assert int(defs["PP_MOMENT_SUMS"]) == _lib.MOMENT_SUMS
+    assert int(defs["PP_MOMENT_CENTRAL"]) == _lib.MOMENT_CENTRAL

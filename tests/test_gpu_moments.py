@@ -1,0 +1,116 @@
+"""Beam moments on the device (pp_beam_moments / ParticleDistribution) against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import pypatools_amd as pa
+from oracle import moments as OM
+from pypatools_amd import _lib
+from pypatools_amd.device import DeviceArray
+from pypatools_amd.engine import ParticleState
+from pypatools_amd.particles import ParticleDistribution
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    from pypatools_amd import device
+    if device.device_count() == 0:
+        pytest.skip("no HIP device visible")
+
+
+def _beam(n, seed):
+    rng = np.random.default_rng(seed)
+    u, w = rng.standard_normal(n), rng.standard_normal(n)
+    x = 2e-3 * u
+    xp = 1e-3 * (w + 0.8 * u)
+    y = 1e-3 * rng.standard_normal(n) + 2e-4
+    yp = 2e-3 * rng.standard_normal(n) - 0.3e-3 * y / 1e-3
+    pz = 0.046 * (1 + 1e-3 * rng.standard_normal(n))
+    return (np.column_stack([x, y, 0.05 + 0.01 * rng.standard_normal(n)]),
+            np.column_stack([xp * pz, yp * pz, pz]))
+
+
+def _check(d, r, p, mass, z_energy=False, rtol=1e-11):
+    ref = OM.moments(r, p, OM.physical_v_from_p(p), mass, z_energy)
+    m = d.recalculate_all()
+    assert m["n"] == len(r)
+    for k in ("centroid", "mean_momentum", "xp_mean", "x_std", "xp_std"):
+        np.testing.assert_allclose(m[k], ref[k], rtol=rtol, atol=1e-300, err_msg=k)
+    for k in ("xxp", "yyp", "xyp", "yxp", "mean_betagamma", "ekin_mean", "ekin_std"):
+        np.testing.assert_allclose(m[k], ref[k], rtol=1e-9, err_msg=k)
+    np.testing.assert_array_equal(m["min"], ref["min"])
+    np.testing.assert_array_equal(m["max"], ref["max"])
+    for mode in ("1rms", "2rms", "full"):
+        np.testing.assert_allclose(d.get_beam_edges(mode), OM.beam_edges(ref, mode), rtol=1e-11)
+    np.testing.assert_allclose(d.get_emittances(), OM.emittances(ref), rtol=1e-8)
+    np.testing.assert_allclose(d.get_emittances(False), OM.emittances(ref, False), rtol=1e-8)
+    tw = d.get_twiss_parameters(report=False)
+    otw = OM.twiss(ref)
+    np.testing.assert_allclose(tw, otw, rtol=1e-7)
+    e = OM.emittances(ref, False)
+    fx, fy = OM.ellipse_fractions(r, OM.physical_v_from_p(p), tw, e[0], e[1])
+    np.testing.assert_allclose(d.inside_4rms_percent, (fx, fy), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 257, 4099, 200_000])
+def test_moments_match_oracle(n):
+    r, p = _beam(n, 10 + n)
+    if n < 3:
+        d = ParticleDistribution(pa.IonSpecies("proton"), x_vec=r, p_vec=p)
+        np.testing.assert_array_equal(d.centroid, r.mean(axis=0))
+        return
+    _check(ParticleDistribution(pa.IonSpecies("proton"), x_vec=r, p_vec=p), r, p, pa.IonSpecies("proton").mass_mev)
+
+
+def test_moments_of_velocity_state_alive_mask_and_z_energy():
+    r, p = _beam(50_000, 21)
+    v = OM.physical_v_from_p(p)
+    sp = pa.IonSpecies("H2_1+")
+    st = ParticleState.from_host(r, v)
+    d = ParticleDistribution.from_state(st, sp, u_state=False)
+    _check(d, r, p, sp.mass_mev, rtol=1e-10)
+    d2 = ParticleDistribution(sp, x_vec=r, p_vec=p, recalculate=False, z_energy=True)
+    _check(d2, r, p, sp.mass_mev, z_energy=True)
+    keep = np.random.default_rng(2).random(len(r)) < 0.6
+    d3 = ParticleDistribution(sp, x_vec=r, p_vec=p, recalculate=False)
+    d3.alive_mask = keep
+    m = d3.recalculate_all()
+    ref = OM.moments(r[keep], p[keep], v[keep], sp.mass_mev)
+    assert m["n"] == keep.sum()
+    np.testing.assert_allclose(m["x_std"], ref["x_std"], rtol=1e-11)
+    np.testing.assert_allclose(m["xxp"], ref["xxp"], rtol=1e-9)
+
+
+def test_moments_deterministic_and_empty():
+    r, p = _beam(300_000, 4)
+    d = ParticleDistribution(x_vec=r, p_vec=p, recalculate=False)
+    a, b = d.recalculate_all(), d.recalculate_all()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+    st = ParticleState(0)
+    sums, cen = DeviceArray((_lib.MOMENT_SUMS,)), DeviceArray((_lib.MOMENT_CENTRAL,))
+    nb = ctypes.c_size_t()
+    _lib.call("pp_beam_workspace_bytes", ctypes.byref(nb))
+    ws = DeviceArray((nb.value,), np.uint8)
+    _lib.call("pp_beam_moments", st._state_args(), 0, 938.0, 0, sums.ptr, cen.ptr, ws.ptr, ws.nbytes, None)
+    assert sums.to_host()[0] == 0.0 and np.all(cen.to_host() == 0.0)
+    with pytest.raises(ValueError):
+        _lib.call("pp_beam_moments", st._state_args(), 0, -1.0, 0, sums.ptr, cen.ptr, ws.ptr, ws.nbytes, None)
+    with pytest.raises(ValueError):
+        _lib.call("pp_beam_moments", st._state_args(), 0, 938.0, 0, sums.ptr, cen.ptr, ws.ptr, 16, None)
+
+
+def test_pusher_bridge_round_trip():
+    """to_pusher / from_pusher convert with the vector form; a drift leaves the emittance unchanged."""
+    r, p = _beam(20_000, 8)
+    d = ParticleDistribution(x_vec=r, p_vec=p)
+    r0, v0 = d.to_pusher()
+    np.testing.assert_allclose(v0, OM.physical_v_from_p(p), rtol=1e-15)
+    pu = pa.Pusher(pa.IonSpecies("proton"), "boris")
+    rh, vh, _ = pu.track_batch(r0, v0, pa.Field.zero(dim=0), pa.Field.zero(dim=0), 200, 1e-10, 200)
+    d2 = ParticleDistribution.from_pusher(rh[-1], vh[-1])
+    np.testing.assert_allclose(d2.p_vec, p, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(d2.get_emittances(), d.get_emittances(), rtol=1e-9)
