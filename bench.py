@@ -99,6 +99,35 @@ def gather_probe(fe, fb, nthreads, iters):
     return res
 
 
+def moments_bandwidth(state, reps=5):
+    """Device beam moments (two reduction passes over the SoA state): time per call and HBM GB/s.
+
+    Algorithmic bytes: 6 float64 per particle per pass, 2 passes.
+    """
+    from pypatools_amd import _lib
+    from pypatools_amd.device import DeviceArray, Event, synchronize
+    import ctypes
+    sums, cen = DeviceArray((_lib.MOMENT_SUMS,)), DeviceArray((_lib.MOMENT_CENTRAL,))
+    nb = ctypes.c_size_t()
+    _lib.call("pp_beam_workspace_bytes", ctypes.byref(nb))
+    ws = DeviceArray((nb.value,), np.uint8)
+    args = state._state_args()
+
+    def call():
+        _lib.call("pp_beam_moments", args, 0, 938.272, 0, sums.ptr, cen.ptr, ws.ptr, ws.nbytes, None)
+
+    call()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    synchronize()
+    t = e0.elapsed_ms(e1) * 1e-3 / reps
+    return {"particles": state.n, "ms_per_call": t * 1e3, "GBps": 2 * 48 * state.n / t / 1e9,
+            "hbm_frac": 2 * 48 * state.n / t / 1e9 / HBM_PEAK_GBS}
+
+
 def cpu_baseline(w, seconds=15.0):
     """The FP64 oracle (canonical vectorised NumPy form, one core) on a sample of C2."""
     from oracle.field import OPeriodicZ
@@ -453,6 +482,12 @@ def main():
                                          "probe": probe}},
         "extra": extra,
     }
+    # beam diagnostics on the resident state: the C2 bunch, and a 4e7-particle state (HBM-resident)
+    big = ParticleState(40_000_000)
+    for arr, val in zip((big.x, big.y, big.z, big.vx, big.vy, big.vz), (1e-3, -2e-3, 0.05, 1e4, -2e4, 2.2e7)):
+        arr.fill(val)
+    out["extra"]["beam_moments"] = [moments_bandwidth(st0), moments_bandwidth(big)]
+    del big
     out["config"]["sort_every"] = args.sort_every
     out["config"]["refine_bits"] = args.refine_bits
     if not args.no_cpu:

@@ -5,8 +5,8 @@
 // Pass 1 sums positions, momenta (beta*gamma), kinetic energy and the angles
 // v / v_z, and takes the position extremes.  Pass 2 takes the means from pass 1
 // and sums centred second moments (no cancellation of sum-of-squares forms).
-// Each pass writes one partial per block; one block then adds the partials in a
-// fixed order, so a result does not depend on scheduling.  A third pass counts
+// Each pass writes one partial per block; one block per quantity then combines
+// the partials in a fixed order, so a result does not depend on scheduling.  A third pass counts
 // the particles inside the 4-rms Twiss ellipses.
 #include "pp_internal.h"
 
@@ -166,18 +166,25 @@ __global__ void __launch_bounds__(kBlock) k_ellipse(const MomentArgs a, const do
     block_reduce<2>(v, op, partial + (size_t)blockIdx.x * 2);
 }
 
-// adds nblocks partials of nq quantities in block order; op as in the passes
-__global__ void __launch_bounds__(64) k_finish(const double* __restrict__ partial, int nblocks, int nq, int which,
-                                               double* __restrict__ out) {
-    const int k = threadIdx.x;
-    if (k >= nq) return;
+// combines the nblocks partials of quantity blockIdx.x: thread t takes blocks t, t+256, ...
+// in order, then a fixed-shape tree over the 256 threads (the result does not depend on timing)
+__global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ partial, int nblocks, int nq, int which,
+                                                   double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    const int k = blockIdx.x, t = threadIdx.x;
     const int op = which == 1 ? kOp1[k] : 0;
-    double s = partial[k];
-    for (int b = 1; b < nblocks; ++b) {
+    double s = op == 0 ? 0.0 : (op < 0 ? INFINITY : -INFINITY);
+    for (int b = t; b < nblocks; b += kBlock) {
         const double o = partial[(size_t)b * nq + k];
         s = op == 0 ? s + o : (op < 0 ? fmin(s, o) : fmax(s, o));
     }
-    out[k] = s;
+    sh[t] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] = op == 0 ? sh[t] + sh[t + w] : (op < 0 ? fmin(sh[t], sh[t + w]) : fmax(sh[t], sh[t + w]));
+        __syncthreads();
+    }
+    if (t == 0) out[k] = sh[0];
 }
 
 }  // namespace pp_beamk
@@ -195,9 +202,11 @@ hipError_t moments(const MomentArgs& a, double* sums, double* central, void* ws,
     double* partial = static_cast<double*>(ws);
     const int nb = blocks_for(a.n);
     hipLaunchKernelGGL(pp_beamk::k_pass1, dim3(nb), dim3(pp_beamk::kBlock), 0, s, a, partial);
-    hipLaunchKernelGGL(pp_beamk::k_finish, dim3(1), dim3(64), 0, s, partial, nb, pp_beamk::kQ1, 1, sums);
+    hipLaunchKernelGGL(pp_beamk::k_finish, dim3(pp_beamk::kQ1), dim3(pp_beamk::kBlock), 0, s, partial, nb,
+                       pp_beamk::kQ1, 1, sums);
     hipLaunchKernelGGL(pp_beamk::k_pass2, dim3(nb), dim3(pp_beamk::kBlock), 0, s, a, sums, partial);
-    hipLaunchKernelGGL(pp_beamk::k_finish, dim3(1), dim3(64), 0, s, partial, nb, pp_beamk::kQ2, 2, central);
+    hipLaunchKernelGGL(pp_beamk::k_finish, dim3(pp_beamk::kQ2), dim3(pp_beamk::kBlock), 0, s, partial, nb,
+                       pp_beamk::kQ2, 2, central);
     return hipGetLastError();
 }
 
@@ -205,7 +214,7 @@ hipError_t ellipse_counts(const MomentArgs& a, const double* twiss_dev, double* 
     double* partial = static_cast<double*>(ws);
     const int nb = blocks_for(a.n);
     hipLaunchKernelGGL(pp_beamk::k_ellipse, dim3(nb), dim3(pp_beamk::kBlock), 0, s, a, twiss_dev, partial);
-    hipLaunchKernelGGL(pp_beamk::k_finish, dim3(1), dim3(64), 0, s, partial, nb, 2, 3, counts);
+    hipLaunchKernelGGL(pp_beamk::k_finish, dim3(2), dim3(pp_beamk::kBlock), 0, s, partial, nb, 2, 3, counts);
     return hipGetLastError();
 }
 

@@ -104,8 +104,11 @@ def test_moments_deterministic_and_empty():
 
 
 def test_pusher_bridge_round_trip():
-    """to_pusher / from_pusher convert with the vector form; a drift leaves the emittance unchanged."""
+    """to_pusher / from_pusher convert with the vector form; a field-free drift leaves the x-x' and y-y'
+    emittances unchanged.  With p_z the same for every particle, v_z differs only at O(x'^2 + y'^2)
+    (~1e-6 here), so the drift is linear in (x, x') to that order (the oracle gives 4e-12)."""
     r, p = _beam(20_000, 8)
+    p[:, 2] = 0.046
     d = ParticleDistribution(x_vec=r, p_vec=p)
     r0, v0 = d.to_pusher()
     np.testing.assert_allclose(v0, OM.physical_v_from_p(p), rtol=1e-15)
@@ -113,4 +116,4 @@ def test_pusher_bridge_round_trip():
     rh, vh, _ = pu.track_batch(r0, v0, pa.Field.zero(dim=0), pa.Field.zero(dim=0), 200, 1e-10, 200)
     d2 = ParticleDistribution.from_pusher(rh[-1], vh[-1])
     np.testing.assert_allclose(d2.p_vec, p, rtol=1e-12, atol=1e-18)
-    np.testing.assert_allclose(d2.get_emittances(), d.get_emittances(), rtol=1e-9)
+    np.testing.assert_allclose(d2.get_emittances()[:2], d.get_emittances()[:2], rtol=1e-8)
