@@ -428,12 +428,18 @@ constexpr int kStageSlots = 10;
 constexpr int kStageWaves = 4;      // wavefronts per block (the hot kernels launch <= 256 threads)
 constexpr int kStagePitch = 13;     // float4 per slot
 
+// RF factor of one field at global step gs and RK stage (wave-uniform: a scalar load)
+DEV double hot_rf(const HotRF& rf, int mode, int64_t gs, int stage) {
+    return rf.table[(gs - rf.step0) * 8 + stage * 2 + (mode == PP_RF_SIN ? 1 : 0)];
+}
+
 template <bool PERIODIC>
 struct HotFE32 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
     HotScale k;
-    DEV void operator()(int64_t, double x, double y, double z, int64_t, int, f3& E, f3& B) const {
+    HotRF rf;
+    DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B) const {
         double tx, ty, tz;
         bool ok;
         const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
@@ -493,8 +499,21 @@ struct HotFE32 {
             for (int c = 0; c < 6; ++c)
                 v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
         }
-        const float me = ok ? k.kE : 0.f, ae = ok ? 0.f : k.kEf;
-        const float mb = ok ? k.kB : 0.f, ab = ok ? 0.f : k.kBf;
+        float kE = k.kE, kEf = k.kEf, kB = k.kB, kBf = k.kBf;
+        if (rf.table) {
+            if (rf.e != PP_RF_NONE) {
+                const float f = (float)hot_rf(rf, rf.e, gs, stage);
+                kE *= f;
+                kEf *= f;
+            }
+            if (rf.b != PP_RF_NONE) {
+                const float f = (float)hot_rf(rf, rf.b, gs, stage);
+                kB *= f;
+                kBf *= f;
+            }
+        }
+        const float me = ok ? kE : 0.f, ae = ok ? 0.f : kEf;
+        const float mb = ok ? kB : 0.f, ab = ok ? 0.f : kBf;
         E = mkf(fmaf(v[0], me, ae), fmaf(v[1], me, ae), fmaf(v[2], me, ae));
         B = mkf(fmaf(v[3], mb, ab), fmaf(v[4], mb, ab), fmaf(v[5], mb, ab));
     }
@@ -504,7 +523,8 @@ template <bool PERIODIC>
 struct HotFE64 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
-    DEV void operator()(int64_t, double x, double y, double z, int64_t, int, d3& E, d3& B) const {
+    HotRF rf;
+    DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, d3& E, d3& B) const {
         double tx, ty, tz;
         bool ok;
         const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
@@ -516,8 +536,13 @@ struct HotFE64 {
             v[c] = cevald(ld2d(rs, o), ld2d(rs, o + 16u), ld2d(rs, o + 32u), ld2d(rs, o + 48u), tx, ty, tz);
         }
         const double fl = m.fill;
-        E = mk3(m.sE * (ok ? v[0] : fl), m.sE * (ok ? v[1] : fl), m.sE * (ok ? v[2] : fl));
-        B = mk3(m.sB * (ok ? v[3] : fl), m.sB * (ok ? v[4] : fl), m.sB * (ok ? v[5] : fl));
+        double sE = m.sE, sB = m.sB;
+        if (rf.table) {
+            if (rf.e != PP_RF_NONE) sE *= hot_rf(rf, rf.e, gs, stage);
+            if (rf.b != PP_RF_NONE) sB *= hot_rf(rf, rf.b, gs, stage);
+        }
+        E = mk3(sE * (ok ? v[0] : fl), sE * (ok ? v[1] : fl), sE * (ok ? v[2] : fl));
+        B = mk3(sB * (ok ? v[3] : fl), sB * (ok ? v[4] : fl), sB * (ok ? v[5] : fl));
     }
 };
 
@@ -553,8 +578,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (p.kick_only) {
         if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
             if constexpr (MIXED)
-                v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC>{p.m, rs, hot_scale(p.m, 1.0)}, ac);
-            else v = kick64<ALG>(i, r, v, dt, p.step0, HotFE64<PERIODIC>{p.m, rs}, ac);
+                v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC>{p.m, rs, hot_scale(p.m, 1.0), p.rf}, ac);
+            else v = kick64<ALG>(i, r, v, dt, p.step0, HotFE64<PERIODIC>{p.m, rs, p.rf}, ac);
             p.vx[i] = v.x;
             p.vy[i] = v.y;
             p.vz[i] = v.z;
@@ -563,14 +588,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     const float h = (float)(0.5 * p.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
-    const HotFE32<PERIODIC> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0)};
+    const HotFE32<PERIODIC> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0), p.rf};
     const bool rec_on = p.r_hist != nullptr;
     int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
-        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC>{p.m, rs}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC>{p.m, rs, p.rf}, ac);
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
             write_record_to(p.r_hist, p.v_hist, p.n, p.n_records, p.ids, rec_idx, i, r, vrec);
@@ -597,10 +622,10 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
     d3 vrec = v;
     const float h = (float)(0.5 * ac.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
-    const HotFE32<PERIODIC> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0)};
+    const HotFE32<PERIODIC> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0), HotRF{}};
     for (int64_t s = 0; s < ns; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, step0 + s, fe32, ac);
-        else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs, HotRF{}}, ac);
     }
 }
 

@@ -404,15 +404,17 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
 }
 
 // The hot kernels cover the common case: E and B from one uniform 3-D map built
-// with E in components 0..2 and B in 3..5, no RF / offset, every particle active,
-// one dt, a single-gather algorithm or RK4, fast or mixed precision.
+// with E in components 0..2 and B in 3..5 (each optionally with an RF factor), no
+// offset, every particle active, one dt, a single-gather algorithm or RK4, fast
+// or mixed precision.
 static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& hp) {
     if (a->precision == PP_PREC_FP64 || !p.fused || a->alive || a->dt_pp) return false;
     if (a->algorithm == PP_ALG_YOSHIDA || a->algorithm == PP_ALG_YOSHIDA_REL) return false;
     const FieldDev& E = p.E;
     const FieldDev& B = p.B;
     const MapDesc& m = E.map;
-    if (E.rf || B.rf || E.has_offset || B.has_offset || E.comp0 != 0 || B.comp0 != 3 || m.ncomp != 6) return false;
+    if ((E.rf || B.rf) && !p.rf_table) return false;
+    if (E.has_offset || B.has_offset || E.comp0 != 0 || B.comp0 != 3 || m.ncomp != 6) return false;
     if (!m.uniform || m.n[0] < 2 || m.n[1] < 2 || m.n[2] < 2) return false;
     const bool mixed = a->precision == PP_PREC_FP32_MIXED;
     const void* coef = mixed ? (const void*)m.coef32 : (const void*)m.coef64;
@@ -437,6 +439,12 @@ static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& 
     hp.n_records = p.n_records;
     hp.kick_only = p.kick_only;
     fill_hot_map(m, coef, bytes, hp.m);
+    if (E.rf || B.rf) {
+        hp.rf.table = p.rf_table;
+        hp.rf.step0 = p.rf_step0;
+        hp.rf.e = E.rf;
+        hp.rf.b = B.rf;
+    }
     hp.m.sE = E.has_scale2 ? E.scale1 * E.scale2 : E.scale1;
     hp.m.sB = B.has_scale2 ? B.scale1 * B.scale2 : B.scale1;
     return true;

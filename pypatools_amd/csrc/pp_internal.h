@@ -78,6 +78,13 @@ struct HotMap {
     uint32_t zbias;         // nc2 * floor(2^30 / nc2): makes the z cell index non-negative before the wrap
 };
 
+// RF factors of the hot kernels: table [step - step0][stage][cos, sin]; e, b = pp_rf_mode of E and B
+struct HotRF {
+    const double* table;   // NULL: static fields
+    int64_t step0;
+    int32_t e, b;
+};
+
 struct HotParams {
     double* x; double* y; double* z;
     double* vx; double* vy; double* vz;
@@ -89,6 +96,7 @@ struct HotParams {
     int32_t kick_only;
     int32_t _pad;
     HotMap m;
+    HotRF rf;
 };
 
 struct BatchParams {

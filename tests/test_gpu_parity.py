@@ -196,6 +196,35 @@ def test_strict_rf_time_dependence_bitwise(alg):
     bitwise(vh, ovh)
 
 
+@pytest.mark.parametrize("alg", ["rk4_rel", "boris", "vay"])
+@pytest.mark.parametrize("prec,tol", [("fp64_fast", 1e-12), ("fp32_mixed", 1e-6)])
+def test_fast_rf_map_matches_strict(alg, prec, tol):
+    """RF maps on the fast kernels (one combined E+B map, cos/sin factor per step and RK stage) against
+    strict FP64, which equals the oracle bit for bit (test_strict_rf_time_dependence_bitwise).
+    5000 particles, so the state is cell-sorted and the gathers take the staged routes."""
+    rng = np.random.default_rng(26)
+    n = 12
+    x = np.linspace(-4e-3, 4e-3, n)
+    z = np.linspace(0, 30e-3, 3 * n)
+    X, Y, Z = np.meshgrid(x, x, z, indexing="ij")
+    k = 2 * np.pi * 1e8 / K.CLIGHT
+    Ez = {"x": 1e5 * X, "y": np.zeros_like(X), "z": 1e7 * (1 - (k * np.hypot(X, Y)) ** 2 / 4)}
+    Bt = {"x": -1e7 / K.CLIGHT * k * Y / 2, "y": 1e7 / K.CLIGHT * k * X / 2, "z": np.zeros_like(X)}
+    grid = {"x": x, "y": x, "z": z}
+    M, N, dt = 5000, 400, 2e-13
+    r0 = np.column_stack([rng.uniform(-3e-3, 3e-3, M), rng.uniform(-3e-3, 3e-3, M), np.full(M, 1e-3)])
+    v0 = np.column_stack([rng.normal(0, 1e5, M), rng.normal(0, 1e5, M), np.full(M, 0.548 * K.CLIGHT)])
+    w, phi = 2 * np.pi * 1e8, 0.4
+    fe = pa.RFField(pa.Field.from_arrays(grid, Ez), w, phi, "cos")
+    fb = pa.RFField(pa.Field.from_arrays(grid, Bt), w, phi, "sin")
+    rh, vh, _ = pa.Pusher(_Ion(QM_E), alg, precision=prec).track_batch(r0, v0, fe, fb, N, dt, N, t0=1e-10)
+    sh, sv, _ = pa.Pusher(_Ion(QM_E), alg).track_batch(r0, v0, fe, fb, N, dt, N, t0=1e-10)
+    path = np.linalg.norm(v0, axis=1) * N * dt
+    assert np.max(np.linalg.norm(rh[-1] - sh[-1], axis=1) / path) < tol
+    assert np.max(np.linalg.norm(vh[-1] - sv[-1], axis=1) / np.linalg.norm(sv[-1], axis=1)) < 10 * tol
+    assert np.max(np.abs(sv[-1, :, 2] - v0[:, 2])) > 1e5     # the RF field did change the beam
+
+
 def test_strict_1d_2d_fields_and_composites():
     rng = np.random.default_rng(17)
     gz = np.linspace(0, 0.02, 21)
