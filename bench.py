@@ -128,6 +128,30 @@ def moments_bandwidth(state, reps=5):
             "hbm_frac": 2 * 48 * state.n / t / 1e9 / HBM_PEAK_GBS}
 
 
+def collision_rate(st_old, st_new, reps=5):
+    """Electrode loss test: every particle's step segment against a 128 x 64 beam-pipe mesh (16384 triangles)."""
+    from pypatools_amd.device import DeviceArray, Event, synchronize
+    from pypatools_amd.electrodes import TriangleElectrodes
+    pipe = TriangleElectrodes.cylinder(0.06, 0.0, 0.128, n_phi=128, n_z=64)
+    n = st_old.n
+    hit = DeviceArray((n,), np.uint8)
+
+    def call():
+        pipe.collide_device((st_old.x.ptr, st_old.y.ptr, st_old.z.ptr), (st_new.x.ptr, st_new.y.ptr, st_new.z.ptr),
+                            n, None, hit.ptr)
+
+    call()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    synchronize()
+    t = e0.elapsed_ms(e1) * 1e-3 / reps
+    return {"segments": n, "triangles": pipe.n_triangles, "ms_per_call": t * 1e3, "segments_per_s": n / t,
+            "hits": int(hit.to_host().sum())}
+
+
 def cpu_baseline(w, seconds=15.0):
     """The FP64 oracle (canonical vectorised NumPy form, one core) on a sample of C2."""
     from oracle.field import OPeriodicZ
@@ -471,7 +495,7 @@ def main():
                    "precision": args.precision, "particles": M, "time_steps_per_bench_step": N},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true> (boris, "
+                     "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true, false> (boris, "
                                f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.2f} ms)",
                      "algorithmic_bytes_per_particle_step": GATHER_BYTES[args.precision],
                      "gflops": FLOPS_PER_STEP * units / (avg_ms * 1e-3) / 1e9,
@@ -488,6 +512,7 @@ def main():
         arr.fill(val)
     out["extra"]["beam_moments"] = [moments_bandwidth(st0), moments_bandwidth(big)]
     del big
+    out["extra"]["electrode_collision"] = collision_rate(st0, st)
     out["config"]["sort_every"] = args.sort_every
     out["config"]["refine_bits"] = args.refine_bits
     if not args.no_cpu:

@@ -319,6 +319,22 @@ int pp_beam_moments(const pp_push_args* state, int32_t u_state, double mass_mev,
 int pp_beam_ellipse_counts(const pp_push_args* state, int32_t u_state, const double* twiss_dev, double* counts_dev,
                            void* workspace, size_t ws_bytes, pp_stream s);
 
+/* Electrode surfaces as a triangle mesh for particle loss (replaces the
+   electrode_assembly.segment_intersects_surface call of Pusher.track_batch,
+   pusher.py:982-990; the reference's py_electrodes meshes are an external package).
+   pp_mesh_create copies host vertices (nv, 3) and vertex-index triangles (nt, 3) to
+   the device with a bounding-volume hierarchy.  pp_collide_segments tests the
+   segments (x0,y0,z0) -> (x1,y1,z1) of slots i < n: a segment that meets a triangle
+   (closed: edges and end points count; a segment in the triangle's plane does not)
+   sets hit[i] = 1 and alive[i] = 0.  Slots with alive[i] == 0 are skipped (hit 0).
+   alive and hit may each be NULL.  Stream-ordered. */
+typedef struct pp_mesh pp_mesh;
+int pp_mesh_create(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, pp_mesh** out);
+int pp_mesh_destroy(pp_mesh* m);
+int pp_mesh_info(const pp_mesh* m, int64_t* ntriangles, int64_t* nnodes);
+int pp_collide_segments(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
+                        const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, pp_stream s);
+
 /* Gather-bandwidth probe (measurement utility; no counterpart in the reference).
    nthreads threads each read `iters` cells of 192 B from the FP32 coefficient table
    of `map` through the same buffer-load path as the FP32 hot kernels and fold them

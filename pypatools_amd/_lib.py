@@ -149,6 +149,11 @@ SIGNATURES = {
     "pp_aos_to_soa": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_gather_probe": (c_int32, [c_void_p, c_int32, c_uint32, c_int32, c_int64, c_void_p, c_void_p]),
     "pp_beam_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_size_t)]),
+    "pp_mesh_create": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, ctypes.POINTER(c_void_p)]),
+    "pp_mesh_destroy": (c_int32, [c_void_p]),
+    "pp_mesh_info": (c_int32, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "pp_collide_segments": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                      c_void_p, c_void_p, c_void_p]),
     "pp_beam_moments": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, ctypes.c_double, c_int32, c_void_p,
                                   c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_beam_ellipse_counts": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, c_void_p, c_void_p, c_void_p,

@@ -19,7 +19,7 @@ OUT = os.path.join(PKG, "libpypat_hip.so")
 BUILD_DIR = os.path.join(PKG, "_build")
 
 ARCH = os.environ.get("PYPAT_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["pp_host.hip", "pp_strict.hip", "pp_fast.hip", "pp_sort.hip", "pp_moments.hip"]
+SOURCES = ["pp_host.hip", "pp_strict.hip", "pp_fast.hip", "pp_sort.hip", "pp_moments.hip", "pp_collide.hip"]
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-I", INCLUDE, "-I", CSRC,
@@ -29,6 +29,7 @@ COMMON = [
 ]
 PER_FILE = {
     "pp_strict.hip": ["-ffp-contract=off"],
+    "pp_collide.hip": ["-ffp-contract=off"],
 }
 
 

@@ -433,7 +433,7 @@ DEV double hot_rf(const HotRF& rf, int mode, int64_t gs, int stage) {
     return rf.table[(gs - rf.step0) * 8 + stage * 2 + (mode == PP_RF_SIN ? 1 : 0)];
 }
 
-template <bool PERIODIC>
+template <bool PERIODIC, bool RF = false>
 struct HotFE32 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
@@ -500,7 +500,7 @@ struct HotFE32 {
                 v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
         }
         float kE = k.kE, kEf = k.kEf, kB = k.kB, kBf = k.kBf;
-        if (rf.table) {
+        if constexpr (RF) {
             if (rf.e != PP_RF_NONE) {
                 const float f = (float)hot_rf(rf, rf.e, gs, stage);
                 kE *= f;
@@ -519,7 +519,7 @@ struct HotFE32 {
     }
 };
 
-template <bool PERIODIC>
+template <bool PERIODIC, bool RF = false>
 struct HotFE64 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
@@ -537,7 +537,7 @@ struct HotFE64 {
         }
         const double fl = m.fill;
         double sE = m.sE, sB = m.sB;
-        if (rf.table) {
+        if constexpr (RF) {
             if (rf.e != PP_RF_NONE) sE *= hot_rf(rf, rf.e, gs, stage);
             if (rf.b != PP_RF_NONE) sB *= hot_rf(rf, rf.b, gs, stage);
         }
@@ -565,7 +565,7 @@ constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || AL
 
 // 4 wavefronts per SIMD (<= 128 VGPRs) keeps enough loads and LDS reads in flight;
 // the compiler would otherwise spend registers for a deeper load pipeline at 3.
-template <int ALG, bool MIXED, bool PERIODIC>
+template <int ALG, bool MIXED, bool PERIODIC, bool RF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
@@ -578,8 +578,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (p.kick_only) {
         if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
             if constexpr (MIXED)
-                v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC>{p.m, rs, hot_scale(p.m, 1.0), p.rf}, ac);
-            else v = kick64<ALG>(i, r, v, dt, p.step0, HotFE64<PERIODIC>{p.m, rs, p.rf}, ac);
+                v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC, RF>{p.m, rs, hot_scale(p.m, 1.0), p.rf},
+                                    ac);
+            else v = kick64<ALG>(i, r, v, dt, p.step0, HotFE64<PERIODIC, RF>{p.m, rs, p.rf}, ac);
             p.vx[i] = v.x;
             p.vy[i] = v.y;
             p.vz[i] = v.z;
@@ -588,14 +589,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     const float h = (float)(0.5 * p.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
-    const HotFE32<PERIODIC> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0), p.rf};
+    const HotFE32<PERIODIC, RF> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0), p.rf};
     const bool rec_on = p.r_hist != nullptr;
     int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
-        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC>{p.m, rs, p.rf}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC, RF>{p.m, rs, p.rf}, ac);
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
             write_record_to(p.r_hist, p.v_hist, p.n, p.n_records, p.ids, rec_idx, i, r, vrec);
@@ -758,14 +759,21 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
     return hipGetLastError();
 }
 
+#define PP_HOT_LAUNCH(A, RFV)                                                                           \
+    if (mixed) {                                                                                        \
+        if (periodic) hipLaunchKernelGGL((k_hot<A, true, true, RFV>), g, b, 0, s, p);                  \
+        else hipLaunchKernelGGL((k_hot<A, true, false, RFV>), g, b, 0, s, p);                          \
+    } else {                                                                                            \
+        if (periodic) hipLaunchKernelGGL((k_hot<A, false, true, RFV>), g, b, 0, s, p);                 \
+        else hipLaunchKernelGGL((k_hot<A, false, false, RFV>), g, b, 0, s, p);                         \
+    }
+
 #define PP_HOT_CASE(A)                                                                                  \
     case A:                                                                                             \
-        if (mixed) {                                                                                    \
-            if (periodic) hipLaunchKernelGGL((k_hot<A, true, true>), g, b, 0, s, p);                   \
-            else hipLaunchKernelGGL((k_hot<A, true, false>), g, b, 0, s, p);                           \
+        if (rf) {                                                                                       \
+            PP_HOT_LAUNCH(A, true)                                                                      \
         } else {                                                                                        \
-            if (periodic) hipLaunchKernelGGL((k_hot<A, false, true>), g, b, 0, s, p);                  \
-            else hipLaunchKernelGGL((k_hot<A, false, false>), g, b, 0, s, p);                          \
+            PP_HOT_LAUNCH(A, false)                                                                     \
         }                                                                                               \
         break;
 
@@ -773,6 +781,7 @@ hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int 
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
     const bool mixed = prec == PP_PREC_FP32_MIXED;
+    const bool rf = p.rf.table != nullptr;
     switch (alg) {
         PP_HOT_CASE(PP_ALG_LEAPFROG)
         PP_HOT_CASE(PP_ALG_BORIS)

@@ -861,6 +861,44 @@ static int util_common(int which, double* a, double* b, double* c, double* aos, 
     return PP_OK;
 }
 
+int pp_mesh_create(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, pp_mesh** out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!out) return fail(PP_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (nv < 0 || nt < 0 || nt > ((int64_t)1 << 30) || nv > ((int64_t)1 << 31) - 1)
+        return fail(PP_ERR_INVALID, "vertex / triangle counts out of range");
+    if ((nv > 0 && !vertices) || (nt > 0 && !triangles)) return fail(PP_ERR_INVALID, "null host array");
+    char err[256] = {0};
+    rc = pp_collide::create(vertices, nv, triangles, nt, out, err, sizeof(err));
+    if (rc) return fail(rc, "%s", err);
+    return PP_OK;
+}
+
+int pp_mesh_destroy(pp_mesh* m) {
+    pp_collide::destroy(m);
+    return PP_OK;
+}
+
+int pp_mesh_info(const pp_mesh* m, int64_t* ntriangles, int64_t* nnodes) {
+    if (!m) return fail(PP_ERR_INVALID, "null mesh");
+    pp_collide::info(m, ntriangles, nnodes);
+    return PP_OK;
+}
+
+int pp_collide_segments(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
+                        const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!m) return fail(PP_ERR_INVALID, "null mesh");
+    if (pp_collide::device_of(m) != g_device) return fail(PP_ERR_INVALID, "mesh lives on another device");
+    if (n < 0) return fail(PP_ERR_INVALID, "negative segment count");
+    if (n == 0) return PP_OK;
+    if (!x0 || !y0 || !z0 || !x1 || !y1 || !z1) return fail(PP_ERR_INVALID, "null device pointer");
+    HIP_TRY(pp_collide::collide(m, x0, y0, z0, x1, y1, z1, n, alive, hit, stream_of(s)), "collision kernel");
+    return PP_OK;
+}
+
 static int moment_args(const pp_push_args* a, int32_t u_state, double mass_mev, int32_t z_energy, MomentArgs& m) {
     if (!a) return fail(PP_ERR_INVALID, "null state");
     if (a->n < 0) return fail(PP_ERR_INVALID, "negative particle count");

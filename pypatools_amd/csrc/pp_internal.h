@@ -175,6 +175,17 @@ hipError_t moments(const MomentArgs& a, double* sums, double* central, void* ws,
 hipError_t ellipse_counts(const MomentArgs& a, const double* twiss_dev, double* counts, void* ws, hipStream_t s);
 }  // namespace pp_beam
 
+struct pp_mesh;
+namespace pp_collide {
+int create(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, pp_mesh** out, char* err,
+           size_t errn);
+void destroy(pp_mesh* m);
+void info(const pp_mesh* m, int64_t* ntris, int64_t* nnodes);
+int device_of(const pp_mesh* m);
+hipError_t collide(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
+                   const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, hipStream_t s);
+}  // namespace pp_collide
+
 namespace pp_sort {
 hipError_t workspace_bytes(int64_t n, size_t* bytes);
 hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,

@@ -441,6 +441,20 @@ class Tracker:
             if elec is not None:
                 raise NotImplementedError("electrode collisions with RK4 need fields the kernels read directly")
             return self._rk4_stage_loop(st, fp, nsteps, dt, t0, hist, dtp, verbose)
+        from .electrodes import TriangleElectrodes
+        if isinstance(elec, TriangleElectrodes) and fp.direct:
+            # loss test on the device: push one step, then the segment of every live slot
+            # against the mesh clears its alive flag (no host round trip per step)
+            old = [DeviceArray((M,)) for _ in range(3)]
+            for step in range(nsteps):
+                if verbose:
+                    _progress(step, nsteps)
+                for o, c in zip(old, (st.x, st.y, st.z)):
+                    o.copy_from(c)
+                self._launch(self._args(st, fp, dt, 1, step, rf=rf, hist=hist, dt_pp=dtp))
+                elec.collide_device(tuple(o.ptr for o in old), (st.x.ptr, st.y.ptr, st.z.ptr), M, st.alive.ptr)
+            active[:] = st.alive.to_host().astype(bool)
+            return
         r_old_host = None
         for step in range(nsteps):
             if verbose:
