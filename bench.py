@@ -152,6 +152,24 @@ def collision_rate(st_old, st_new, reps=5):
             "hits": int(hit.to_host().sum())}
 
 
+def space_charge_timing(state, cells=(128, 128, 128), extent=(0.128, 0.128, 0.128), reps=3):
+    """Device space-charge step of the resident bunch: CIC deposit, direct Poisson solve, E = -grad(phi)."""
+    from pypatools_amd.device import Event, synchronize
+    from pypatools_amd.space_charge import SpaceChargeSolver
+    s = SpaceChargeSolver(extent, cells)
+    q = 1e-9 / state.n                                  # a 1 nC bunch
+    s._solve_device(state.x.ptr, state.y.ptr, state.z.ptr, None, q, state.n)
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(reps):
+        s._solve_device(state.x.ptr, state.y.ptr, state.z.ptr, None, q, state.n)
+    e1.record()
+    synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    phi = s.phi.to_host()
+    return {"particles": state.n, "mesh": list(cells), "ms_per_solve": ms, "phi_max_V": float(phi.max())}
+
+
 def cpu_baseline(w, seconds=15.0):
     """The FP64 oracle (canonical vectorised NumPy form, one core) on a sample of C2."""
     from oracle.field import OPeriodicZ
@@ -513,6 +531,7 @@ def main():
     out["extra"]["beam_moments"] = [moments_bandwidth(st0), moments_bandwidth(big)]
     del big
     out["extra"]["electrode_collision"] = collision_rate(st0, st)
+    out["extra"]["space_charge"] = space_charge_timing(st0)
     out["config"]["sort_every"] = args.sort_every
     out["config"]["refine_bits"] = args.refine_bits
     if not args.no_cpu:

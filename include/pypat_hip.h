@@ -335,6 +335,26 @@ int pp_mesh_info(const pp_mesh* m, int64_t* ntriangles, int64_t* nnodes);
 int pp_collide_segments(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
                         const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, pp_stream s);
 
+/* Space charge on a box mesh without electrodes (PyAMGPoissonSolver.solve,
+   poisson_amg.py:616-797): nx*ny*nz cell-centre nodes spanning the extent
+   (Lx, Ly, Lz) centred on the origin, spacing h = L / n, phi = 0 one spacing
+   outside the box.  Device arrays are (nx, ny, nz) float64, z fastest.
+   pp_cic_deposit adds the cloud-in-cell charge density of particles (positions
+   x, y, z; charge q[i], or q_const when q is NULL; slots with alive[i] == 0
+   skipped) to rho [C/m^3] (poisson_amg.py:849-911; rho is not cleared).
+   pp_poisson_solve: phi with -lap(phi) = rho / eps0 for the reference's 7-point
+   operator (poisson_amg.py:293-444 without conductors), solved directly by sine
+   transforms.  pp_efield_from_potential: E = -grad(phi) by central differences,
+   0 on the outer node layer (poisson_amg.py:98-130).  All stream-ordered. */
+typedef struct pp_poisson pp_poisson;
+int pp_poisson_create(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz, pp_poisson** out);
+int pp_poisson_destroy(pp_poisson* p);
+int pp_cic_deposit(const pp_poisson* p, const double* x, const double* y, const double* z, const double* q,
+                   double q_const, const uint8_t* alive, int64_t n, double* rho, pp_stream s);
+int pp_poisson_solve(const pp_poisson* p, const double* rho, double* phi, pp_stream s);
+int pp_efield_from_potential(const pp_poisson* p, const double* phi, double* ex, double* ey, double* ez,
+                             pp_stream s);
+
 /* Gather-bandwidth probe (measurement utility; no counterpart in the reference).
    nthreads threads each read `iters` cells of 192 B from the FP32 coefficient table
    of `map` through the same buffer-load path as the FP32 hot kernels and fold them

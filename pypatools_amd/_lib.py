@@ -149,6 +149,13 @@ SIGNATURES = {
     "pp_aos_to_soa": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_gather_probe": (c_int32, [c_void_p, c_int32, c_uint32, c_int32, c_int64, c_void_p, c_void_p]),
     "pp_beam_workspace_bytes": (c_int32, [ctypes.POINTER(ctypes.c_size_t)]),
+    "pp_poisson_create": (c_int32, [c_int32, c_int32, c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.POINTER(c_void_p)]),
+    "pp_poisson_destroy": (c_int32, [c_void_p]),
+    "pp_cic_deposit": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double, c_void_p, c_int64,
+                                 c_void_p, c_void_p]),
+    "pp_poisson_solve": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pp_efield_from_potential": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_mesh_create": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, ctypes.POINTER(c_void_p)]),
     "pp_mesh_destroy": (c_int32, [c_void_p]),
     "pp_mesh_info": (c_int32, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),

@@ -19,7 +19,8 @@ OUT = os.path.join(PKG, "libpypat_hip.so")
 BUILD_DIR = os.path.join(PKG, "_build")
 
 ARCH = os.environ.get("PYPAT_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["pp_host.hip", "pp_strict.hip", "pp_fast.hip", "pp_sort.hip", "pp_moments.hip", "pp_collide.hip"]
+SOURCES = ["pp_host.hip", "pp_strict.hip", "pp_fast.hip", "pp_sort.hip", "pp_moments.hip", "pp_collide.hip",
+           "pp_poisson.hip"]
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-I", INCLUDE, "-I", CSRC,
@@ -30,6 +31,7 @@ COMMON = [
 PER_FILE = {
     "pp_strict.hip": ["-ffp-contract=off"],
     "pp_collide.hip": ["-ffp-contract=off"],
+    "pp_poisson.hip": ["-ffp-contract=off"],
 }
 
 

@@ -861,6 +861,70 @@ static int util_common(int which, double* a, double* b, double* c, double* aos, 
     return PP_OK;
 }
 
+int pp_poisson_create(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz, pp_poisson** out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!out) return fail(PP_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (nx < 3 || ny < 3 || nz < 3 || (int64_t)nx * ny * nz > ((int64_t)1 << 28))
+        return fail(PP_ERR_INVALID, "mesh cells must be >= 3 per axis and at most 2^28 in total");
+    if (!(lx > 0.0) || !(ly > 0.0) || !(lz > 0.0) || !std::isfinite(lx) || !std::isfinite(ly) || !std::isfinite(lz))
+        return fail(PP_ERR_INVALID, "domain extent must be positive and finite");
+    PoissonDesc d;
+    memset(&d, 0, sizeof(d));
+    d.n[0] = nx; d.n[1] = ny; d.n[2] = nz;
+    d.L[0] = lx; d.L[1] = ly; d.L[2] = lz;
+    d.h[0] = lx / nx; d.h[1] = ly / ny; d.h[2] = lz / nz;     // poisson_amg.py:181-183
+    char err[256] = {0};
+    rc = pp_poisson_impl::create(d, out, err, sizeof(err));
+    if (rc) return fail(rc, "%s", err);
+    return PP_OK;
+}
+
+int pp_poisson_destroy(pp_poisson* p) {
+    pp_poisson_impl::destroy(p);
+    return PP_OK;
+}
+
+static int poisson_check(const pp_poisson* p) {
+    if (!p) return fail(PP_ERR_INVALID, "null solver");
+    if (pp_poisson_impl::device_of(p) != g_device) return fail(PP_ERR_INVALID, "solver lives on another device");
+    return PP_OK;
+}
+
+int pp_cic_deposit(const pp_poisson* p, const double* x, const double* y, const double* z, const double* q,
+                   double q_const, const uint8_t* alive, int64_t n, double* rho, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (n < 0) return fail(PP_ERR_INVALID, "negative particle count");
+    if (!rho) return fail(PP_ERR_INVALID, "null rho");
+    if (n == 0) return PP_OK;
+    if (!x || !y || !z) return fail(PP_ERR_INVALID, "null device pointer");
+    HIP_TRY(pp_poisson_impl::deposit(p, x, y, z, q, q_const, alive, n, rho, stream_of(s)), "CIC deposit");
+    return PP_OK;
+}
+
+int pp_poisson_solve(const pp_poisson* p, const double* rho, double* phi, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (!rho || !phi) return fail(PP_ERR_INVALID, "null device pointer");
+    if (rho == phi) return fail(PP_ERR_INVALID, "rho and phi must be distinct arrays");
+    HIP_TRY(pp_poisson_impl::solve(p, rho, phi, stream_of(s)), "Poisson solve");
+    return PP_OK;
+}
+
+int pp_efield_from_potential(const pp_poisson* p, const double* phi, double* ex, double* ey, double* ez,
+                             pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (!phi || !ex || !ey || !ez) return fail(PP_ERR_INVALID, "null device pointer");
+    HIP_TRY(pp_poisson_impl::gradient(p, phi, ex, ey, ez, stream_of(s)), "field from potential");
+    return PP_OK;
+}
+
 int pp_mesh_create(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, pp_mesh** out) {
     int rc = ensure_init();
     if (rc) return rc;

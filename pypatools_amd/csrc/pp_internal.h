@@ -175,6 +175,28 @@ hipError_t moments(const MomentArgs& a, double* sums, double* central, void* ws,
 hipError_t ellipse_counts(const MomentArgs& a, const double* twiss_dev, double* counts, void* ws, hipStream_t s);
 }  // namespace pp_beam
 
+// box mesh of the space-charge solve: n cell-centre nodes per axis, spacing h = L / n
+struct PoissonDesc {
+    int32_t n[3];
+    int32_t _pad;
+    double h[3];
+    double L[3];
+};
+
+constexpr double kEps0 = 8.8541878188e-12;   // vacuum permittivity (scipy.constants.epsilon_0, CODATA 2022)
+
+struct pp_poisson;
+namespace pp_poisson_impl {
+int create(const PoissonDesc& d, pp_poisson** out, char* err, size_t errn);
+void destroy(pp_poisson* p);
+const PoissonDesc& desc(const pp_poisson* p);
+int device_of(const pp_poisson* p);
+hipError_t deposit(const pp_poisson* p, const double* x, const double* y, const double* z, const double* q,
+                   double qc, const uint8_t* alive, int64_t n, double* rho, hipStream_t s);
+hipError_t solve(const pp_poisson* p, const double* rho, double* phi, hipStream_t s);
+hipError_t gradient(const pp_poisson* p, const double* phi, double* ex, double* ey, double* ez, hipStream_t s);
+}  // namespace pp_poisson_impl
+
 struct pp_mesh;
 namespace pp_collide {
 int create(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, pp_mesh** out, char* err,
