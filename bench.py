@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--graph", action="store_true", help="replay the C4 window loop from a HIP graph")
     ap.add_argument("--jobs", type=int, default=300)
     ap.add_argument("--load", type=float, default=0.7, help="C5 offered load as a fraction of --capacity")
-    ap.add_argument("--capacity", type=float, default=1.0e11, help="C5 capacity (particle-steps/s) for lambda")
+    ap.add_argument("--capacity", type=float, default=0.0,
+                    help="C5 capacity (particle-steps/s) for lambda; 0: measure it on a closed batch of C5 jobs")
     ap.add_argument("--pool", type=int, default=60_000_000, help="C5 particle pool (HBM budget)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the AoS->SoA copy kernel over this many MiB (counter calibration) and exit")
@@ -349,6 +350,20 @@ def bench_c5(args):
     srv = TrackingServer(fields, args.precision, chunk_steps=256, pool_particles=args.pool)
     mean_work = 2.63e5 * 3.76e3       # E[N] E[steps] of the C5 distributions
     capacity = args.capacity
+    measured = None
+    if capacity <= 0:
+        # capacity of the server on the C5 job mix (Boris / Vay / RK4, which costs 4 gathers per step):
+        # a closed batch of C5-distributed jobs submitted at once and run to completion
+        cal = c5_jobs(max(args.jobs // 5, 40), 1e9, seed=11)
+        t = time.perf_counter()
+        for j in cal:
+            srv.submit(j)
+        srv.run_until_idle()
+        el = time.perf_counter() - t
+        measured = sum(j.n * j.nsteps for j in cal) / el
+        capacity = measured
+        srv.particle_steps = 0
+        srv.rounds = 0
     lam = args.load * capacity / mean_work
     jobs = c5_jobs(args.jobs, lam)
     t0 = time.perf_counter()
@@ -371,13 +386,15 @@ def bench_c5(args):
            "ms_per_step": span * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32+f64" if args.precision == "fp32_mixed" else "f64", "data": "synthetic",
            "config": {"workload": f"C5: {len(jobs)} jobs, Poisson arrivals at {lam:.2f} jobs/s "
-                                  f"({args.load:.0%} of {capacity:.2e} particle-steps/s capacity), N log-uniform "
+                                  f"({args.load:.0%} of {capacity:.2e} particle-steps/s "
+                                  f"{'measured ' if measured else ''}capacity), N log-uniform "
                                   "1e3-2e6, steps log-uniform 1e2-2e4, Boris/Vay/RK4, 4 resident 128^3 maps, "
                                   "256-step chunks", "precision": args.precision, "pool_particles": srv.pool_n},
            "extra": {"p50_s": float(np.percentile(lat, 50)), "p90_s": float(np.percentile(lat, 90)),
                      "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
                      "throughput_particle_steps_per_s": work / span, "makespan_s": span, "rounds": srv.rounds,
-                     "arrival_rate_jobs_per_s": lam}}
+                     "arrival_rate_jobs_per_s": lam, "capacity_particle_steps_per_s": capacity,
+                     "capacity_measured": measured is not None}}
     print(json.dumps(out), flush=True)
 
 
