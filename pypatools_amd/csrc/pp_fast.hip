@@ -468,14 +468,16 @@ struct HotFE32 {
             uint64_t todo = ~0ull;
             uint32_t myslot = 0, ld0 = 0, ld1 = 0;
             int ns = 0;
-            while (todo != 0 && ns < kStageSlots) {
+#pragma unroll
+            for (int it = 0; it < kStageSlots; ++it) {
+                if (todo == 0) break;
                 const uint32_t c = __builtin_amdgcn_readlane(cell, (int)__builtin_ctzll(todo));
                 const bool mine = cell == c;
                 todo &= ~__builtin_amdgcn_ballot_w64(mine);
-                myslot = mine ? (uint32_t)ns : myslot;
-                if (ns < 5) ld0 = lslot == (uint32_t)ns ? c : ld0;
-                else ld1 = lslot == (uint32_t)(ns - 5) ? c : ld1;
-                ++ns;
+                myslot = mine ? (uint32_t)it : myslot;
+                if (it < 5) ld0 = lslot == (uint32_t)it ? c : ld0;
+                else ld1 = lslot == (uint32_t)(it - 5) ? c : ld1;
+                ns = it + 1;
             }
             if (todo == 0) {
                 float4* wv = stage[threadIdx.x >> 6];
