@@ -153,6 +153,42 @@ def collision_rate(st_old, st_new, reps=5):
             "hits": int(hit.to_host().sum())}
 
 
+def lossy_tracking(w, fp, qm, nsteps=2000, radius=3e-3, precision="fp32_mixed"):
+    """The C2 bunch inside a beam pipe along its whole path: the loss test fused into the step loop
+    (pp_push_lossy), with compaction of the live slots every 500 steps (partition + cell sort), and
+    without compaction; against the same steps with no mesh (hot kernel)."""
+    from pypatools_amd.device import DeviceArray, synchronize
+    from pypatools_amd.electrodes import TriangleElectrodes
+    from pypatools_amd.engine import ParticleState, Tracker
+    dt = w["dt"]
+    length = float(np.max(w["v0"][:, 2])) * nsteps * dt + 0.2
+    pipe = TriangleElectrodes.cylinder(radius, 0.0, length, n_phi=64, n_z=256)
+    M = len(w["r0"])
+    res = {"particles": M, "steps": nsteps, "pipe_radius_m": radius, "pipe_length_m": length,
+           "triangles": pipe.n_triangles, "precision": precision}
+    for sort_every in (500, 0):
+        st = ParticleState.from_host(w["r0"], w["v0"])
+        st.alive = DeviceArray.from_host(np.ones(M, dtype=np.uint8))
+        tr = Tracker("boris", precision, qm, sort_every=sort_every)
+        active = np.ones(M, dtype=bool)
+        synchronize()
+        t = time.perf_counter()
+        tr._lossy_loop(st, fp, nsteps, dt, None, None, pipe, active, None, False)
+        synchronize()
+        el = time.perf_counter() - t
+        key = "compacted" if sort_every else "not_compacted"
+        res[key] = {"s": el, "particle_steps_per_s": M * nsteps / el, "lost_fraction": float(1 - active.mean())}
+    st = ParticleState.from_host(w["r0"], w["v0"])
+    tr = Tracker("boris", precision, qm)
+    synchronize()
+    t = time.perf_counter()
+    tr.run_resident(st, fp, nsteps, dt)
+    synchronize()
+    el = time.perf_counter() - t
+    res["no_mesh"] = {"s": el, "particle_steps_per_s": M * nsteps / el}
+    return res
+
+
 def space_charge_timing(state, cells=(128, 128, 128), extent=(0.128, 0.128, 0.128), reps=3):
     """Device space-charge step of the resident bunch: CIC deposit, direct Poisson solve, E = -grad(phi)."""
     from pypatools_amd.device import Event, synchronize
@@ -548,6 +584,7 @@ def main():
     out["extra"]["beam_moments"] = [moments_bandwidth(st0), moments_bandwidth(big)]
     del big
     out["extra"]["electrode_collision"] = collision_rate(st0, st)
+    out["extra"]["lossy_tracking"] = lossy_tracking(w, fp, qm)
     out["extra"]["space_charge"] = space_charge_timing(st0)
     out["config"]["sort_every"] = args.sort_every
     out["config"]["refine_bits"] = args.refine_bits

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define PP_ABI_VERSION 1
+#define PP_ABI_VERSION 2
 
 enum pp_status {
     PP_OK = 0,
@@ -128,6 +128,9 @@ typedef struct pp_push_args {
     int32_t kick_only;       /* 1: velocity-only kick with this dt (Boris staggering,
                                 pusher.py:946-958, 992-1007); nsteps must be 1 */
     int32_t block_size;      /* threads per block: 64, 128, 192 or 256; 0 = library default (256) */
+    int64_t hist_rows;       /* particles per history record (row stride; rows are ids[i] when ids is
+                                given): 0 = n.  Set it to the original count when n covers a
+                                compacted live prefix (pp_compact_state) */
 } pp_push_args;
 
 /* ---- runtime ------------------------------------------------------------ */
@@ -273,7 +276,8 @@ int pp_add_i64(int64_t* dptr, int64_t value, pp_stream s);   /* *dptr += value o
    locality to manage).  Reorders the SoA state of `a` (x..vz) by the map cell of
    each particle -- efield's map, else bfield's -- and permutes `ids` alongside,
    so a wavefront's gathers hit the same cache lines.  pp_unsort_state restores
-   the original order (ids -> 0..n-1).  Workspace: pp_sort_workspace_bytes(n).
+   the original order (ids -> 0..n-1), of args->alive too when it is given.
+   Workspace: pp_sort_workspace_bytes(n).
    No-op when neither field is a map. */
 int pp_sort_workspace_bytes(int64_t n, size_t* bytes);
 int pp_sort_state(const pp_push_args* a, int32_t* ids, void* workspace, size_t ws_bytes, pp_stream s);
@@ -334,6 +338,27 @@ int pp_mesh_destroy(pp_mesh* m);
 int pp_mesh_info(const pp_mesh* m, int64_t* ntriangles, int64_t* nnodes);
 int pp_collide_segments(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
                         const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, pp_stream s);
+
+/* Fused push with the loss test inside the step loop: Pusher.track_batch with an
+   electrode assembly (pusher.py:966-990, push -> record -> segment test -> active
+   update).  Each live slot (args->alive[i] != 0, required) takes up to nsteps steps;
+   after every step, and after its history record, the segment r_old -> r_new is
+   tested against the mesh as in pp_collide_segments, and a hit clears alive[i] and
+   ends that slot's steps (its state is the post-step state, as in the reference).
+   Same arguments and results as pp_push otherwise; never takes the hot kernels. */
+int pp_push_lossy(const pp_push_args* args, const pp_mesh* mesh, pp_stream s);
+
+/* Alive-mask compaction (the live subset r_current[active] of pusher.py:972-973,
+   kept resident): a stable partition of slots [0, args->n) into live slots first and
+   dead slots after, each part ordered by map cell (with sub-cell bins as in
+   pp_sort_state_ex) when `map`, or a map field of args, is given.  The six state
+   arrays and ids are permuted together; args->alive (required) is rewritten to
+   1 on [0, *n_live) and 0 after; *n_live (host) receives the live count, and the
+   call waits for the stream.  Later pushes may then run over the first *n_live
+   slots only; pp_unsort_state over all args->n slots restores the original order.
+   Workspace: pp_sort_workspace_bytes(args->n). */
+int pp_compact_state(const pp_push_args* args, const pp_map* map, int32_t refine_axis, int32_t refine_bits,
+                     int32_t* ids, int64_t* n_live, void* workspace, size_t ws_bytes, pp_stream s);
 
 /* Space charge on a box mesh without electrodes (PyAMGPoissonSolver.solve,
    poisson_amg.py:616-797): nx*ny*nz cell-centre nodes spanning the extent

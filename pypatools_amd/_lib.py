@@ -48,7 +48,7 @@ class PPPushArgs(ctypes.Structure):
         ("efield", PPField), ("bfield", PPField),
         ("rf_table", c_void_p), ("rf_step0", c_int64), ("rf_nsteps", c_int64),
         ("r_hist", c_void_p), ("v_hist", c_void_p), ("rec_every", c_int64), ("n_records", c_int64),
-        ("kick_only", c_int32), ("block_size", c_int32),
+        ("kick_only", c_int32), ("block_size", c_int32), ("hist_rows", c_int64),
     ]
 
 
@@ -161,6 +161,9 @@ SIGNATURES = {
     "pp_mesh_info": (c_int32, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "pp_collide_segments": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
                                       c_void_p, c_void_p, c_void_p]),
+    "pp_push_lossy": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_void_p]),
+    "pp_compact_state": (c_int32, [ctypes.POINTER(PPPushArgs), c_void_p, c_int32, c_int32, c_void_p,
+                                   ctypes.POINTER(c_int64), c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_beam_moments": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, ctypes.c_double, c_int32, c_void_p,
                                   c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_beam_ellipse_counts": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, c_void_p, c_void_p, c_void_p,

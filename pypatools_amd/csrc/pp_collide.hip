@@ -3,77 +3,24 @@
 // the reference's electrode_assembly.segment_intersects_surface call in
 // Pusher.track_batch (src/PyPATools/pusher.py:982-990).
 //
-// The mesh is held as a bounding-volume hierarchy built on the host: nodes in
-// depth-first order, each with the index that follows its subtree ("escape"),
-// so the device walk needs no stack and every step moves to a higher node index
-// (it always ends).  Node boxes are grown by a margin far above rounding, so they
-// only cull; the hit decision is the FP64 segment-triangle test (Moller-Trumbore,
-// closed intervals, no FMA contraction) and matches a brute-force test over every
-// triangle bit for bit.
+// The host builds the bounding-volume hierarchy (median split on the longest
+// centroid axis, leaves of at most four triangles) and a pyramid of candidate
+// grids that names, for each small block of space, the subtree holding every
+// triangle near it; pp_bvh.inc holds the device layouts, the candidate lookup and
+// the walk, shared with the push kernels that test losses inside the step loop.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
 #include <vector>
 
 #include "pp_internal.h"
+#include "pp_bvh.inc"
 
 #pragma clang fp contract(off)
 
 namespace pp_collidek {
 
-struct Node {
-    double lo[3], hi[3];
-    int32_t first, count;   // leaf: triangles [first, first + count); inner node: count == 0
-    int32_t escape, _pad;   // next node index after this subtree
-};
-
-struct Tri {
-    double v0[3], e1[3], e2[3];   // e1 = v1 - v0, e2 = v2 - v0
-};
-
-__device__ __forceinline__ bool box_hit(const Node& b, const double p[3], const double d[3]) {
-    double t0 = 0.0, t1 = 1.0;
-    for (int a = 0; a < 3; ++a) {
-        if (d[a] == 0.0) {
-            if (p[a] < b.lo[a] || p[a] > b.hi[a]) return false;
-        } else {
-            const double inv = 1.0 / d[a];
-            double ta = (b.lo[a] - p[a]) * inv, tb = (b.hi[a] - p[a]) * inv;
-            if (ta > tb) {
-                const double s = ta;
-                ta = tb;
-                tb = s;
-            }
-            t0 = ta > t0 ? ta : t0;
-            t1 = tb < t1 ? tb : t1;
-            if (!(t0 <= t1)) return false;
-        }
-    }
-    return true;
-}
-
-// segment p + t d, t in [0, 1], against one triangle (edges included)
-__device__ __forceinline__ bool tri_hit(const Tri& t, const double p[3], const double d[3]) {
-    const double px = d[1] * t.e2[2] - d[2] * t.e2[1];
-    const double py = d[2] * t.e2[0] - d[0] * t.e2[2];
-    const double pz = d[0] * t.e2[1] - d[1] * t.e2[0];
-    const double det = t.e1[0] * px + t.e1[1] * py + t.e1[2] * pz;
-    if (det == 0.0) return false;
-    const double inv = 1.0 / det;
-    const double sx = p[0] - t.v0[0], sy = p[1] - t.v0[1], sz = p[2] - t.v0[2];
-    const double u = (sx * px + sy * py + sz * pz) * inv;
-    if (u < 0.0 || u > 1.0) return false;
-    const double qx = sy * t.e1[2] - sz * t.e1[1];
-    const double qy = sz * t.e1[0] - sx * t.e1[2];
-    const double qz = sx * t.e1[1] - sy * t.e1[0];
-    const double w = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
-    if (w < 0.0 || u + w > 1.0) return false;
-    const double s = (t.e2[0] * qx + t.e2[1] * qy + t.e2[2] * qz) * inv;
-    return s >= 0.0 && s <= 1.0;
-}
-
-__global__ void __launch_bounds__(256) k_collide(const Node* __restrict__ nodes, int32_t nnodes,
-                                                 const Tri* __restrict__ tris, const double* __restrict__ x0,
+__global__ void __launch_bounds__(256) k_collide(const pp_bvh::MeshDev* md, const double* __restrict__ x0,
                                                  const double* __restrict__ y0, const double* __restrict__ z0,
                                                  const double* __restrict__ x1, const double* __restrict__ y1,
                                                  const double* __restrict__ z1, int64_t n, uint8_t* alive,
@@ -84,22 +31,7 @@ __global__ void __launch_bounds__(256) k_collide(const Node* __restrict__ nodes,
         if (hit) hit[i] = 0;
         return;
     }
-    const double p[3] = {x0[i], y0[i], z0[i]};
-    const double d[3] = {x1[i] - p[0], y1[i] - p[1], z1[i] - p[2]};
-    bool h = false;
-    int32_t node = 0;
-    while (node < nnodes) {
-        const Node& b = nodes[node];
-        if (!box_hit(b, p, d)) {
-            node = b.escape;
-        } else if (b.count == 0) {
-            node = node + 1;
-        } else {
-            for (int32_t k = 0; k < b.count && !h; ++k) h = tri_hit(tris[b.first + k], p, d);
-            if (h) break;
-            node = b.escape;
-        }
-    }
+    const bool h = md ? pp_bvh::mesh_hits(md, x0[i], y0[i], z0[i], x1[i], y1[i], z1[i]) : false;
     if (hit) hit[i] = h ? 1 : 0;
     if (h && alive) alive[i] = 0;
 }
@@ -107,17 +39,20 @@ __global__ void __launch_bounds__(256) k_collide(const Node* __restrict__ nodes,
 }  // namespace pp_collidek
 
 struct pp_mesh {
-    pp_collidek::Node* nodes;
-    pp_collidek::Tri* tris;
+    pp_bvh::Node* nodes;
+    pp_bvh::Tri* tris;
+    pp_bvh::Range* ranges;  // candidate grids, all levels
+    pp_bvh::MeshDev* dev;   // device copy of the descriptor
     int32_t nnodes;
     int64_t ntris;
+    int64_t range_cells;
     int device;
 };
 
 namespace pp_collide {
 
-using pp_collidek::Node;
-using pp_collidek::Tri;
+using pp_bvh::Node;
+using pp_bvh::Tri;
 
 namespace {
 
@@ -128,13 +63,16 @@ struct Builder {
     std::vector<double> lo, hi, cen;   // per triangle, 3 each
     std::vector<int32_t> order;
     std::vector<Node> nodes;
+    std::vector<int32_t> parent, depth;
     double margin;
 
     explicit Builder(const std::vector<Tri>& t) : tri(t) {}
 
-    int32_t build(int32_t b, int32_t e) {
+    int32_t build(int32_t b, int32_t e, int32_t up = -1, int32_t dep = 0) {
         const int32_t me = (int32_t)nodes.size();
         nodes.push_back(Node{});
+        parent.push_back(up);
+        depth.push_back(dep);
         Node nd{};
         for (int a = 0; a < 3; ++a) {
             nd.lo[a] = INFINITY;
@@ -166,8 +104,8 @@ struct Builder {
                              [&](int32_t u, int32_t v) { return cen[3 * u + axis] < cen[3 * v + axis]; });
             nd.first = 0;
             nd.count = 0;
-            build(b, mid);
-            build(mid, e);
+            build(b, mid, me, dep + 1);
+            build(mid, e, me, dep + 1);
         }
         nd.escape = (int32_t)nodes.size();
         nodes[me] = nd;
@@ -175,7 +113,111 @@ struct Builder {
     }
 };
 
+// Candidate grids over the mesh box [glo, ghi].  Level 0 has about 64 cells per
+// triangle (at most 2^22 cells, at most 4096 per axis); level L halves level L-1 on
+// every axis (cell index >> L).  A cell's node is the lowest common ancestor of the
+// leaves of every triangle whose box, grown by the margin, overlaps the cell; each
+// level stores, per cell, the node range [s, e) of that ancestor's subtree, or an
+// empty range.
+struct Candidates {
+    int32_t n0[3] = {1, 1, 1};
+    double lo[3] = {0, 0, 0}, inv[3] = {0, 0, 0};
+    int32_t nlevels = 0;
+    int64_t off[pp_bvh::kMaxLevels] = {};
+    std::vector<pp_bvh::Range> ranges;
+
+    int32_t cell(int a, double x) const {
+        const double f = std::floor((x - lo[a]) * inv[a]);
+        if (!(f >= 0.0)) return 0;
+        return f > (double)(n0[a] - 1) ? n0[a] - 1 : (int32_t)f;
+    }
+
+    static int32_t lca(const Builder& bld, int32_t a, int32_t b) {
+        if (a < 0) return b;
+        if (b < 0) return a;
+        while (a != b) {
+            if (bld.depth[a] > bld.depth[b]) a = bld.parent[a];
+            else if (bld.depth[b] > bld.depth[a]) b = bld.parent[b];
+            else {
+                a = bld.parent[a];
+                b = bld.parent[b];
+            }
+        }
+        return a;
+    }
+
+    void build(const double glo[3], const double ghi[3], const Builder& bld, int64_t nt) {
+        double ext[3], emax = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            ext[a] = ghi[a] - glo[a];
+            emax = std::max(emax, ext[a]);
+        }
+        const double target = (double)std::min<int64_t>(std::max<int64_t>(64 * nt, 4096), (int64_t)1 << 22);
+        double vol = 1.0;
+        for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], 1e-6 * emax);
+        double h = std::cbrt(vol / target);
+        for (int tries = 0; tries < 64; ++tries, h *= 1.25) {
+            int64_t total = 1;
+            for (int a = 0; a < 3; ++a) {
+                n0[a] = ext[a] > 0.0 && h > 0.0 ? (int32_t)std::min(4096.0, std::max(1.0, std::ceil(ext[a] / h))) : 1;
+                total *= n0[a];
+            }
+            if (total <= ((int64_t)1 << 23)) break;
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = glo[a];
+            inv[a] = ext[a] > 0.0 ? (double)n0[a] / ext[a] : 0.0;
+        }
+        // leaf of each triangle (positions in the sorted triangle order)
+        std::vector<int32_t> leaf_of((size_t)nt, 0);
+        for (int32_t v = 0; v < (int32_t)bld.nodes.size(); ++v)
+            if (bld.nodes[v].count > 0)
+                for (int32_t k = 0; k < bld.nodes[v].count; ++k) leaf_of[bld.nodes[v].first + k] = v;
+        // level 0: common ancestor of the triangles near each cell
+        std::vector<int32_t> cur((size_t)n0[0] * n0[1] * n0[2], -1);
+        for (int64_t k = 0; k < nt; ++k) {
+            const int32_t t = bld.order[k];
+            int32_t c0[3], c1[3];
+            for (int a = 0; a < 3; ++a) {
+                c0[a] = cell(a, bld.lo[3 * t + a] - bld.margin);
+                c1[a] = cell(a, bld.hi[3 * t + a] + bld.margin);
+            }
+            for (int32_t i = c0[0]; i <= c1[0]; ++i)
+                for (int32_t j = c0[1]; j <= c1[1]; ++j)
+                    for (int32_t q = c0[2]; q <= c1[2]; ++q) {
+                        int32_t& c = cur[((size_t)i * n0[1] + j) * n0[2] + q];
+                        c = lca(bld, c, leaf_of[k]);
+                    }
+        }
+        int32_t n[3] = {n0[0], n0[1], n0[2]};
+        ranges.clear();
+        for (nlevels = 0; nlevels < pp_bvh::kMaxLevels; ++nlevels) {
+            off[nlevels] = (int64_t)ranges.size();
+            for (const int32_t v : cur) ranges.push_back(v >= 0 ? pp_bvh::Range{v, bld.nodes[v].escape}
+                                                                : pp_bvh::Range{0, 0});
+            if (n[0] == 1 && n[1] == 1 && n[2] == 1) {
+                ++nlevels;
+                break;
+            }
+            // next level: cell index >> 1 on every axis
+            int32_t m[3];
+            for (int a = 0; a < 3; ++a) m[a] = ((n[a] - 1) >> 1) + 1;
+            std::vector<int32_t> nxt((size_t)m[0] * m[1] * m[2], -1);
+            for (int32_t i = 0; i < n[0]; ++i)
+                for (int32_t j = 0; j < n[1]; ++j)
+                    for (int32_t q = 0; q < n[2]; ++q) {
+                        int32_t& c = nxt[((size_t)(i >> 1) * m[1] + (j >> 1)) * m[2] + (q >> 1)];
+                        c = lca(bld, c, cur[((size_t)i * n[1] + j) * n[2] + q]);
+                    }
+            cur.swap(nxt);
+            for (int a = 0; a < 3; ++a) n[a] = m[a];
+        }
+    }
+};
+
 }  // namespace
+
+void destroy(pp_mesh* m);
 
 int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh** out, char* err, size_t errn) {
     std::vector<Tri> tri((size_t)nt);
@@ -218,20 +260,44 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
     if (nt > 0) bld.build(0, (int32_t)nt);
     std::vector<Tri> sorted((size_t)nt);
     for (int64_t k = 0; k < nt; ++k) sorted[k] = tri[bld.order[k]];
+    Candidates cand;
+    if (nt > 0) cand.build(glo, ghi, bld, nt);
     pp_mesh* m = new pp_mesh();
     m->nnodes = (int32_t)bld.nodes.size();
     m->ntris = nt;
     m->nodes = nullptr;
     m->tris = nullptr;
+    m->ranges = nullptr;
+    m->dev = nullptr;
+    m->range_cells = nt > 0 ? (int64_t)cand.ranges.size() : 0;
     hipGetDevice(&m->device);
     if (m->nnodes > 0) {
-        if (hipMalloc(&m->nodes, sizeof(Node) * m->nnodes) != hipSuccess ||
-            hipMalloc(&m->tris, sizeof(Tri) * (size_t)nt) != hipSuccess ||
-            hipMemcpy(m->nodes, bld.nodes.data(), sizeof(Node) * m->nnodes, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->tris, sorted.data(), sizeof(Tri) * (size_t)nt, hipMemcpyHostToDevice) != hipSuccess) {
-            if (m->nodes) (void)hipFree(m->nodes);
-            if (m->tris) (void)hipFree(m->tris);
-            delete m;
+        pp_bvh::MeshDev d{};
+        const size_t rbytes = sizeof(pp_bvh::Range) * cand.ranges.size();
+        bool ok = hipMalloc(&m->nodes, sizeof(Node) * m->nnodes) == hipSuccess &&
+                  hipMalloc(&m->tris, sizeof(Tri) * (size_t)nt) == hipSuccess &&
+                  hipMalloc(&m->ranges, rbytes) == hipSuccess &&
+                  hipMalloc(&m->dev, sizeof(pp_bvh::MeshDev)) == hipSuccess;
+        if (ok) {
+            d.nodes = m->nodes;
+            d.tris = m->tris;
+            d.ranges = m->ranges;
+            d.nnodes = m->nnodes;
+            d.nlevels = cand.nlevels;
+            for (int a = 0; a < 3; ++a) {
+                d.n0[a] = cand.n0[a];
+                d.lo[a] = cand.lo[a];
+                d.inv[a] = cand.inv[a];
+            }
+            for (int l = 0; l < pp_bvh::kMaxLevels; ++l) d.off[l] = cand.off[l];
+            d.margin = bld.margin;
+            ok = hipMemcpy(m->nodes, bld.nodes.data(), sizeof(Node) * m->nnodes, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(m->tris, sorted.data(), sizeof(Tri) * (size_t)nt, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(m->ranges, cand.ranges.data(), rbytes, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(m->dev, &d, sizeof(d), hipMemcpyHostToDevice) == hipSuccess;
+        }
+        if (!ok) {
+            destroy(m);
             snprintf(err, errn, "device allocation for the mesh failed");
             return PP_ERR_NOMEM;
         }
@@ -244,6 +310,8 @@ void destroy(pp_mesh* m) {
     if (!m) return;
     if (m->nodes) (void)hipFree(m->nodes);
     if (m->tris) (void)hipFree(m->tris);
+    if (m->ranges) (void)hipFree(m->ranges);
+    if (m->dev) (void)hipFree(m->dev);
     delete m;
 }
 
@@ -254,11 +322,14 @@ void info(const pp_mesh* m, int64_t* ntris, int64_t* nnodes) {
 
 int device_of(const pp_mesh* m) { return m->device; }
 
+const void* device_view(const pp_mesh* m) { return m->dev; }
+
 hipError_t collide(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
                    const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, hipStream_t s) {
     const unsigned grid = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(pp_collidek::k_collide, dim3(grid), dim3(256), 0, s, m->nodes, m->nnodes, m->tris, x0, y0, z0,
-                       x1, y1, z1, n, alive, hit);
+    // an empty mesh has no descriptor (md == null): nothing is hit
+    hipLaunchKernelGGL(pp_collidek::k_collide, dim3(grid), dim3(256), 0, s, m->dev, x0, y0, z0, x1, y1, z1, n, alive,
+                       hit);
     return hipGetLastError();
 }
 

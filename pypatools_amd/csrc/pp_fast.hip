@@ -7,6 +7,7 @@
 //   - position, velocity (or u = gamma v) accumulated in FP64; gamma - 1 from
 //     (u^2/c^2) / (1 + gamma) so it keeps full relative precision at low energy.
 #include "pp_internal.h"
+#include "pp_bvh.inc"
 
 #define PP_STRICT 0
 
@@ -239,7 +240,7 @@ DEV d3 kick_mixed(int64_t i, d3 r, d3 v, double dt, int64_t gs, const FE& fe, co
     else return vay_kick(v, Ed, Bd, dt, p.qm, p.c2, p.clight);
 }
 
-template <int ALG>
+template <int ALG, bool LOSS>
 __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
@@ -264,11 +265,18 @@ __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
+        const d3 r0 = r;
         step_mixed<ALG>(i, r, v, vrec, dt, h, p.step0 + s, fe, ac);
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
             write_record(p, rec_idx, i, r, vrec);
             ++rec_idx;
+        }
+        if constexpr (LOSS) {
+            if (lost_on_segment(p, r0, r)) {
+                p.alive[i] = 0;
+                break;
+            }
         }
     }
     p.x[i] = r.x;
@@ -736,27 +744,36 @@ hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int blo
     return hipGetLastError();
 }
 
-#define PP_FAST_CASES(KERNEL)                                                                             \
+#define PP_FAST_CASES(KERNEL, L)                                                                          \
     switch (alg) {                                                                                        \
-        case PP_ALG_LEAPFROG: hipLaunchKernelGGL(KERNEL<PP_ALG_LEAPFROG>, g, b, 0, s, p); break;          \
-        case PP_ALG_BORIS: hipLaunchKernelGGL(KERNEL<PP_ALG_BORIS>, g, b, 0, s, p); break;                \
-        case PP_ALG_RK4: hipLaunchKernelGGL(KERNEL<PP_ALG_RK4>, g, b, 0, s, p); break;                    \
-        case PP_ALG_YOSHIDA: hipLaunchKernelGGL(KERNEL<PP_ALG_YOSHIDA>, g, b, 0, s, p); break;            \
-        case PP_ALG_VAY_REL: hipLaunchKernelGGL(KERNEL<PP_ALG_VAY_REL>, g, b, 0, s, p); break;            \
-        case PP_ALG_RK4_REL: hipLaunchKernelGGL(KERNEL<PP_ALG_RK4_REL>, g, b, 0, s, p); break;            \
-        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL(KERNEL<PP_ALG_YOSHIDA_REL>, g, b, 0, s, p); break;    \
-        case PP_ALG_BORIS_REL: hipLaunchKernelGGL(KERNEL<PP_ALG_BORIS_REL>, g, b, 0, s, p); break;        \
-        case PP_ALG_VAY: hipLaunchKernelGGL(KERNEL<PP_ALG_VAY>, g, b, 0, s, p); break;                    \
+        case PP_ALG_LEAPFROG: hipLaunchKernelGGL((KERNEL<PP_ALG_LEAPFROG, L>), g, b, 0, s, p); break;     \
+        case PP_ALG_BORIS: hipLaunchKernelGGL((KERNEL<PP_ALG_BORIS, L>), g, b, 0, s, p); break;           \
+        case PP_ALG_RK4: hipLaunchKernelGGL((KERNEL<PP_ALG_RK4, L>), g, b, 0, s, p); break;               \
+        case PP_ALG_YOSHIDA: hipLaunchKernelGGL((KERNEL<PP_ALG_YOSHIDA, L>), g, b, 0, s, p); break;       \
+        case PP_ALG_VAY_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_VAY_REL, L>), g, b, 0, s, p); break;       \
+        case PP_ALG_RK4_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_RK4_REL, L>), g, b, 0, s, p); break;       \
+        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_YOSHIDA_REL, L>), g, b, 0, s, p); break; \
+        case PP_ALG_BORIS_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_BORIS_REL, L>), g, b, 0, s, p); break;   \
+        case PP_ALG_VAY: hipLaunchKernelGGL((KERNEL<PP_ALG_VAY, L>), g, b, 0, s, p); break;               \
         default: return hipErrorInvalidValue;                                                             \
     }
 
 hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStream_t s) {
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
+    const bool loss = p.mesh != nullptr;
     if (prec == PP_PREC_FP64_FAST) {
-        PP_FAST_CASES(k_push64)
+        if (loss) {
+            PP_FAST_CASES(k_push64, true)
+        } else {
+            PP_FAST_CASES(k_push64, false)
+        }
     } else {
-        PP_FAST_CASES(k_push_mixed)
+        if (loss) {
+            PP_FAST_CASES(k_push_mixed, true)
+        } else {
+            PP_FAST_CASES(k_push_mixed, false)
+        }
     }
     return hipGetLastError();
 }

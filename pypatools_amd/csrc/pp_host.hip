@@ -409,6 +409,7 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
 // or mixed precision.
 static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& hp) {
     if (a->precision == PP_PREC_FP64 || !p.fused || a->alive || a->dt_pp) return false;
+    if (p.hist_rows != p.n) return false;   // the hot kernels write records of n rows
     if (a->algorithm == PP_ALG_YOSHIDA || a->algorithm == PP_ALG_YOSHIDA_REL) return false;
     const FieldDev& E = p.E;
     const FieldDev& B = p.B;
@@ -469,10 +470,15 @@ int pp_field_eval(const pp_field* field, int32_t precision, const double* pts, i
     return PP_OK;
 }
 
-int pp_push(const pp_push_args* a, pp_stream s) {
+static int push_impl(const pp_push_args* a, const pp_mesh* mesh, pp_stream s) {
     int rc = ensure_init();
     if (rc) return rc;
     if (!a) return fail(PP_ERR_INVALID, "null args");
+    if (mesh) {
+        if (!a->alive) return fail(PP_ERR_INVALID, "a loss test needs the alive mask");
+        if (a->kick_only) return fail(PP_ERR_INVALID, "kick_only moves no particle: no loss test");
+        if (pp_collide::device_of(mesh) != g_device) return fail(PP_ERR_INVALID, "mesh lives on another device");
+    }
     if (a->algorithm < PP_ALG_LEAPFROG || a->algorithm > PP_ALG_VAY)
         return fail(PP_ERR_INVALID, "unknown algorithm %d", a->algorithm);
     if (a->precision < PP_PREC_FP64 || a->precision > PP_PREC_FP32_MIXED)
@@ -493,6 +499,9 @@ int pp_push(const pp_push_args* a, pp_stream s) {
     }
     if (a->block_size < 0 || a->block_size > 256 || (a->block_size % 64) != 0)
         return fail(PP_ERR_INVALID, "block_size must be 64, 128, 192 or 256 (or 0)");
+    if (a->hist_rows != 0 && a->hist_rows < a->n)
+        return fail(PP_ERR_INVALID, "hist_rows (%lld) must be 0 or at least n (%lld)", (long long)a->hist_rows,
+                    (long long)a->n);
     PushParams p;
     memset(&p, 0, sizeof(p));
     if ((rc = make_field(a->efield, a->precision, p.E))) return rc;
@@ -527,6 +536,7 @@ int pp_push(const pp_push_args* a, pp_stream s) {
     p.ids = a->ids;
     p.rec_every = a->r_hist ? a->rec_every : 1;
     p.n_records = a->r_hist ? a->n_records : 0;
+    p.hist_rows = a->hist_rows > 0 ? a->hist_rows : a->n;
     p.clight = kClight;
     p.c2 = kClight * kClight;     // CLIGHT ** 2 (Python float pow == c*c for this value)
     p.vclamp = 0.9999 * kClight;  // pusher.py:276
@@ -534,6 +544,7 @@ int pp_push(const pp_push_args* a, pp_stream s) {
         p.yc[k] = kYoshidaC[k];
         p.yd[k] = kYoshidaD[k];
     }
+    if (mesh) p.mesh = pp_collide::device_view(mesh);   // null for an empty mesh: nothing to hit
     HotParams hp;
     hipError_t e;
     if (hot_eligible(a, p, hp))
@@ -544,6 +555,13 @@ int pp_push(const pp_push_args* a, pp_stream s) {
         e = pp_fast::launch_push(a->algorithm, a->precision, p, a->block_size, stream_of(s));
     if (e != hipSuccess) return hip_fail(e, "push launch");
     return PP_OK;
+}
+
+int pp_push(const pp_push_args* a, pp_stream s) { return push_impl(a, nullptr, s); }
+
+int pp_push_lossy(const pp_push_args* a, const pp_mesh* mesh, pp_stream s) {
+    if (!mesh) return fail(PP_ERR_INVALID, "null mesh");
+    return push_impl(a, mesh, s);
 }
 
 int pp_batch_tiles(const pp_bunch* bunches, int32_t nbunch, int32_t tile, int32_t* tile_bunch, int64_t* tile_start,
@@ -717,7 +735,7 @@ static int sort_common(const pp_push_args* a, int32_t* ids, void* ws, size_t ws_
     if (!ws || ws_bytes < need) return fail(PP_ERR_INVALID, "sort workspace of %zu bytes needed", need);
     double* arrs[6] = {a->x, a->y, a->z, a->vx, a->vy, a->vz};
     if (unsort) {
-        HIP_TRY(pp_sort::unsort_state(arrs, ids, a->n, ws, ws_bytes, stream_of(s)), "unsort");
+        HIP_TRY(pp_sort::unsort_state(arrs, ids, a->n, ws, ws_bytes, stream_of(s), a->alive), "unsort");
         return PP_OK;
     }
     const MapDesc* m = sort_map(a);
@@ -733,6 +751,34 @@ int pp_sort_state(const pp_push_args* a, int32_t* ids, void* ws, size_t ws_bytes
 
 int pp_unsort_state(const pp_push_args* a, int32_t* ids, void* ws, size_t ws_bytes, pp_stream s) {
     return sort_common(a, ids, ws, ws_bytes, s, true);
+}
+
+int pp_compact_state(const pp_push_args* a, const pp_map* map, int32_t refine_axis, int32_t refine_bits,
+                     int32_t* ids, int64_t* n_live, void* ws, size_t ws_bytes, pp_stream s) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!a || !ids || !n_live) return fail(PP_ERR_INVALID, "null argument");
+    if (!a->alive) return fail(PP_ERR_INVALID, "compaction needs the alive mask");
+    if (a->n < 0 || a->n > INT32_MAX) return fail(PP_ERR_INVALID, "compaction supports 0 <= n < 2^31");
+    *n_live = 0;
+    if (a->n == 0) return PP_OK;
+    if (!a->x || !a->y || !a->z || !a->vx || !a->vy || !a->vz) return fail(PP_ERR_INVALID, "null state pointer");
+    if (refine_axis < 0 || refine_axis > 2 || refine_bits < 0 || refine_bits > 8)
+        return fail(PP_ERR_INVALID, "refine_axis must be 0..2 and refine_bits 0..8");
+    const MapDesc* m = map ? &map->d : sort_map(a);
+    if (m && pp_sort::key_bits_of(*m) + refine_bits > 31)
+        return fail(PP_ERR_UNSUPPORTED, "map too large for a cell key with the dead flag in 32 bits");
+    size_t need = 0;
+    HIP_TRY(pp_sort::workspace_bytes(a->n, &need), "sort workspace query");
+    if (!ws || ws_bytes < need) return fail(PP_ERR_INVALID, "sort workspace of %zu bytes needed", need);
+    double* arrs[6] = {a->x, a->y, a->z, a->vx, a->vy, a->vz};
+    int64_t* slot = pp_sort::result_slot(ws, a->n);
+    const hipStream_t st = stream_of(s);
+    HIP_TRY(pp_sort::compact_state(m, arrs, ids, a->alive, a->n, ws, ws_bytes, st, refine_axis, refine_bits, slot),
+            "compaction");
+    HIP_TRY(hipMemcpyAsync(n_live, slot, sizeof(int64_t), hipMemcpyDeviceToHost, st), "live count copy");
+    HIP_TRY(hipStreamSynchronize(st), "compaction sync");
+    return PP_OK;
 }
 
 int pp_iota_i32(int32_t* ids, int64_t n, pp_stream s) {

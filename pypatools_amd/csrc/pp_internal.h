@@ -57,7 +57,10 @@ struct PushParams {
     int64_t rf_step0;
     double* r_hist; double* v_hist;
     int64_t rec_every, n_records;
+    int64_t hist_rows;      // particles per history record (>= n)
     const int32_t* ids;     // original particle index per slot (sorted state), or null
+    // electrode mesh of the loss test inside the step loop (device pp_bvh::MeshDev); null: no test
+    const void* mesh;
     // constants computed on the host with the reference's Python expressions
     double clight, c2, vclamp;
     double yc[3], yd[3];    // Yoshida coefficients, pusher.py:176-181
@@ -204,6 +207,7 @@ int create(const double* vertices, int64_t nv, const int32_t* triangles, int64_t
 void destroy(pp_mesh* m);
 void info(const pp_mesh* m, int64_t* ntris, int64_t* nnodes);
 int device_of(const pp_mesh* m);
+const void* device_view(const pp_mesh* m);   // device pp_bvh::MeshDev, or null for an empty mesh
 hipError_t collide(const pp_mesh* m, const double* x0, const double* y0, const double* z0, const double* x1,
                    const double* y1, const double* z1, int64_t n, uint8_t* alive, uint8_t* hit, hipStream_t s);
 }  // namespace pp_collide
@@ -213,8 +217,17 @@ hipError_t workspace_bytes(int64_t n, size_t* bytes);
 hipError_t sort_state(const MapDesc& m, double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes,
                       hipStream_t s, const int32_t* group = nullptr, int group_shift = 0, int refine_axis = 2,
                       int refine_bits = 0);
-hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes, hipStream_t s);
+// alive (optional) is scattered back with the state
+hipError_t unsort_state(double* arrs[6], int32_t* ids, int64_t n, void* ws, size_t ws_bytes, hipStream_t s,
+                        uint8_t* alive = nullptr);
 hipError_t iota(int32_t* ids, int64_t n, hipStream_t s);
+// Stable partition of the state into live slots (front) and dead slots (back), each
+// part ordered by map cell when m is given; alive is rewritten to 1...1 0...0 and the
+// live count is left in *n_live_dev (device memory).
+hipError_t compact_state(const MapDesc* m, double* arrs[6], int32_t* ids, uint8_t* alive, int64_t n, void* ws,
+                         size_t ws_bytes, hipStream_t s, int refine_axis, int refine_bits, int64_t* n_live_dev);
+int key_bits_of(const MapDesc& m);              // bits of the cell key of map m
+int64_t* result_slot(void* ws, int64_t n);      // 256-byte result area inside a sort workspace of n slots
 }  // namespace pp_sort
 
 enum {

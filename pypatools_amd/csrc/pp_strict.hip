@@ -1,6 +1,7 @@
 // Strict FP64 kernels: no FMA contraction, one IEEE operation per NumPy
 // operation of the reference, so every result equals the FP64 oracle bit for bit.
 #include "pp_internal.h"
+#include "pp_bvh.inc"
 
 #pragma clang fp contract(off)
 
@@ -130,20 +131,27 @@ __global__ void __launch_bounds__(256) k_build_coef(const MapDesc d, double* __r
 
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+#define PP_STRICT_CASES(L)                                                                                \
+    switch (alg) {                                                                                        \
+        case PP_ALG_LEAPFROG: hipLaunchKernelGGL((k_push64<PP_ALG_LEAPFROG, L>), g, b, 0, s, p); break;   \
+        case PP_ALG_BORIS: hipLaunchKernelGGL((k_push64<PP_ALG_BORIS, L>), g, b, 0, s, p); break;         \
+        case PP_ALG_RK4: hipLaunchKernelGGL((k_push64<PP_ALG_RK4, L>), g, b, 0, s, p); break;             \
+        case PP_ALG_YOSHIDA: hipLaunchKernelGGL((k_push64<PP_ALG_YOSHIDA, L>), g, b, 0, s, p); break;     \
+        case PP_ALG_VAY_REL: hipLaunchKernelGGL((k_push64<PP_ALG_VAY_REL, L>), g, b, 0, s, p); break;     \
+        case PP_ALG_RK4_REL: hipLaunchKernelGGL((k_push64<PP_ALG_RK4_REL, L>), g, b, 0, s, p); break;     \
+        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL((k_push64<PP_ALG_YOSHIDA_REL, L>), g, b, 0, s, p); break; \
+        case PP_ALG_BORIS_REL: hipLaunchKernelGGL((k_push64<PP_ALG_BORIS_REL, L>), g, b, 0, s, p); break; \
+        case PP_ALG_VAY: hipLaunchKernelGGL((k_push64<PP_ALG_VAY, L>), g, b, 0, s, p); break;             \
+        default: return hipErrorInvalidValue;                                                             \
+    }
+
 hipError_t launch_push(int alg, const PushParams& p, int block, hipStream_t s) {
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
-    switch (alg) {
-        case PP_ALG_LEAPFROG: hipLaunchKernelGGL(k_push64<PP_ALG_LEAPFROG>, g, b, 0, s, p); break;
-        case PP_ALG_BORIS: hipLaunchKernelGGL(k_push64<PP_ALG_BORIS>, g, b, 0, s, p); break;
-        case PP_ALG_RK4: hipLaunchKernelGGL(k_push64<PP_ALG_RK4>, g, b, 0, s, p); break;
-        case PP_ALG_YOSHIDA: hipLaunchKernelGGL(k_push64<PP_ALG_YOSHIDA>, g, b, 0, s, p); break;
-        case PP_ALG_VAY_REL: hipLaunchKernelGGL(k_push64<PP_ALG_VAY_REL>, g, b, 0, s, p); break;
-        case PP_ALG_RK4_REL: hipLaunchKernelGGL(k_push64<PP_ALG_RK4_REL>, g, b, 0, s, p); break;
-        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL(k_push64<PP_ALG_YOSHIDA_REL>, g, b, 0, s, p); break;
-        case PP_ALG_BORIS_REL: hipLaunchKernelGGL(k_push64<PP_ALG_BORIS_REL>, g, b, 0, s, p); break;
-        case PP_ALG_VAY: hipLaunchKernelGGL(k_push64<PP_ALG_VAY>, g, b, 0, s, p); break;
-        default: return hipErrorInvalidValue;
+    if (p.mesh) {
+        PP_STRICT_CASES(true)
+    } else {
+        PP_STRICT_CASES(false)
     }
     return hipGetLastError();
 }

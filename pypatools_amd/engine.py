@@ -132,11 +132,30 @@ class ParticleState:
             _lib.call("pp_iota_i32", self.ids.ptr, self.n, None)
         self.sorted = False
 
+    def compact(self, fp, refine_axis=2, refine_bits=0):
+        """Live slots first (ordered by map cell when a field is a map), dead slots after;
+        the alive mask becomes 1...1 0...0.  Returns the live count."""
+        if self.n == 0:
+            return 0
+        ids = self.ensure_ids()
+        ws = self._workspace()
+        a = self._state_args(fp)
+        a.alive = self.alive.ptr
+        n_live = ctypes.c_int64()
+        _lib.call("pp_compact_state", a, None, int(refine_axis), int(refine_bits), ids.ptr, ctypes.byref(n_live),
+                  ws.ptr, ws.nbytes, None)
+        self.sorted = True
+        return int(n_live.value)
+
     def unsort(self):
+        """Back to the original order (the alive mask too)."""
         if self.n == 0 or not getattr(self, "sorted", False):
             return
         ws = self._workspace()
-        _lib.call("pp_unsort_state", self._state_args(), self.ids.ptr, ws.ptr, ws.nbytes, None)
+        a = self._state_args()
+        if self.alive is not None:
+            a.alive = self.alive.ptr
+        _lib.call("pp_unsort_state", a, self.ids.ptr, ws.ptr, ws.nbytes, None)
         self.sorted = False
 
     def v_to_u(self):
@@ -443,18 +462,7 @@ class Tracker:
             return self._rk4_stage_loop(st, fp, nsteps, dt, t0, hist, dtp, verbose)
         from .electrodes import TriangleElectrodes
         if isinstance(elec, TriangleElectrodes) and fp.direct:
-            # loss test on the device: push one step, then the segment of every live slot
-            # against the mesh clears its alive flag (no host round trip per step)
-            old = [DeviceArray((M,)) for _ in range(3)]
-            for step in range(nsteps):
-                if verbose:
-                    _progress(step, nsteps)
-                for o, c in zip(old, (st.x, st.y, st.z)):
-                    o.copy_from(c)
-                self._launch(self._args(st, fp, dt, 1, step, rf=rf, hist=hist, dt_pp=dtp))
-                elec.collide_device(tuple(o.ptr for o in old), (st.x.ptr, st.y.ptr, st.z.ptr), M, st.alive.ptr)
-            active[:] = st.alive.to_host().astype(bool)
-            return
+            return self._lossy_loop(st, fp, nsteps, dt, rf, hist, elec, active, dtp, verbose)
         r_old_host = None
         for step in range(nsteps):
             if verbose:
@@ -474,6 +482,39 @@ class Tracker:
                 st.alive.upload(active.astype(np.uint8))
                 if not np.any(active):
                     break
+
+    def _lossy_loop(self, st, fp, nsteps, dt, rf, hist, elec, active, dtp, verbose):
+        """track_batch with an electrode assembly (pusher.py:966-990) on the device.
+
+        Fused launches test every step's segment against the mesh inside the step
+        loop (pp_push_lossy).  Every sort_every steps the state is compacted: live
+        slots first, ordered by map cell, dead slots after (pp_compact_state), and
+        the following launches cover the live prefix only.  Particles never
+        interact, so the order of slots changes no result; history rows follow ids.
+        """
+        M = st.n
+        compact = dtp is None and self.sort_every > 0
+        if compact:
+            st.ensure_ids()
+        n_live, step, since = M, 0, None
+        while step < nsteps and n_live > 0:
+            if compact and (since is None or since >= self.sort_every):
+                n_live = st.compact(fp, self.refine_axis, self.refine_bits)
+                since = 0
+                if n_live == 0:
+                    break
+            k = min(max(1, self.launch_work // n_live), nsteps - step)
+            if compact:
+                k = min(k, self.sort_every - since)
+                since += k
+            if verbose:
+                _progress(step, nsteps)
+            a = self._args(st, fp, dt, k, step, rf=rf, hist=hist, dt_pp=dtp)
+            a.n, a.hist_rows = n_live, M
+            _lib.call("pp_push_lossy", a, elec.handle, None)
+            step += k
+        st.unsort()
+        active[:] = st.alive.to_host().astype(bool)
 
     def _rk4_stage_loop(self, st, fp, nsteps, dt, t0, hist, dtp, verbose):
         """_rk4_step_batch (pusher.py:803-892) with fields evaluated between device stage kernels."""

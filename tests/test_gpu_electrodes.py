@@ -50,6 +50,40 @@ def test_segments_match_brute_force(kind):
     assert not np.any(got[:50] & ~want[:50])
 
 
+@pytest.mark.parametrize("kind", ["box", "cylinder", "soup"])
+def test_short_segments_near_surfaces_match_brute_force(kind):
+    """Segments of 1e-7 .. 1e-2 of the mesh size around the surface (the occupancy grid decides most of
+    them; endpoints on vertices, on edges and exactly on faces included) agree with the brute force."""
+    rng = np.random.default_rng(53)
+    if kind == "box":
+        m = TriangleElectrodes.box([-0.5, -0.4, -0.3], [0.6, 0.5, 0.7])
+    elif kind == "cylinder":
+        m = TriangleElectrodes.cylinder(0.6, -1.0, 1.0, n_phi=48, n_z=12)
+    else:
+        m = TriangleElectrodes(*_soup(rng))
+    v, t = m.vertices, m.triangles
+    n = 20000
+    k = rng.integers(0, len(t), n)
+    bary = rng.dirichlet([1, 1, 1], n)
+    bary[: n // 10] = np.eye(3)[rng.integers(0, 3, n // 10)]             # on a vertex
+    e = slice(n // 10, n // 5)
+    bary[e, 0] = 0.0
+    bary[e] /= bary[e].sum(axis=1, keepdims=True)                       # on an edge
+    on = np.einsum("nk,nkd->nd", bary, v[t[k]])
+    size = float(np.max(v.max(axis=0) - v.min(axis=0)))
+    length = size * 10.0 ** rng.uniform(-7, -2, n)
+    d = rng.standard_normal((n, 3))
+    d *= (length / np.linalg.norm(d, axis=1))[:, None]
+    frac = rng.uniform(-1.5, 0.5, n)                                    # where the surface point sits on the segment
+    r0 = on + frac[:, None] * d
+    r1 = r0 + d
+    r1[: n // 20] = on[: n // 20]                                       # ending exactly on the surface point
+    got = m.segment_intersects_surface(r0, r1)["hit_mask"]
+    want = segment_hits(v, t, r0, r1)
+    np.testing.assert_array_equal(got, want)
+    assert 0.05 < want.mean() < 0.95
+
+
 def test_mesh_validation():
     with pytest.raises(ValueError):
         TriangleElectrodes([[0, 0, 0], [1, 0, 0]], [[0, 1, 2]])
@@ -86,3 +120,40 @@ def test_track_with_device_losses_equals_host_callback_and_oracle(alg):
     np.testing.assert_array_equal(rh, orh)
     np.testing.assert_array_equal(vh, ovh)
     assert 0.05 < 1 - act.mean() < 0.99                  # some, not all, particles were lost
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_mixed"])
+@pytest.mark.parametrize("alg", ["boris", "rk4_rel", "vay"])
+def test_lossy_track_through_a_map_with_compaction(alg, prec):
+    """A beam pipe inside a 3-D map: the fused loss test with compaction every 7 steps (30 compactions,
+    each a partition plus cell sort) equals the host-callback route step by step, histories and active
+    masks bit for bit; in strict FP64 it also equals the oracle pusher with the oracle mesh."""
+    rng = np.random.default_rng(52)
+    n = 5000
+    x = (np.arange(16) - 7.5) * 1e-3
+    z = np.arange(16) * 1e-3 - 2e-3
+    grid = {"x": x, "y": x, "z": z}
+    X, Y, Z = np.meshgrid(x, x, z, indexing="ij")
+    E = {"x": 3e5 * X / 7.5e-3, "y": 3e5 * Y / 7.5e-3, "z": 1e5 + 0 * Z}
+    B = {"x": 0.02 * Y / 7.5e-3, "y": 0.02 * X / 7.5e-3, "z": 0.4 + 0.1 * np.cos(Z / 2e-3)}
+    fe, fb = pa.Field.from_arrays(grid, E), pa.Field.from_arrays(grid, B)
+    pipe = TriangleElectrodes.cylinder(5e-3, -2e-3, 13e-3, n_phi=36, n_z=15)
+    orc_e = OracleElectrodes(pipe.vertices, pipe.triangles)
+    r0 = np.column_stack([rng.uniform(-3e-3, 3e-3, n), rng.uniform(-3e-3, 3e-3, n), rng.uniform(0, 2e-3, n)])
+    v0 = np.column_stack([rng.normal(0, 4e5, n), rng.normal(0, 4e5, n), rng.uniform(1e5, 6e5, n)])
+    ion = pa.IonSpecies("proton")
+    dev = pa.Pusher(ion, alg, precision=prec, electrode_assembly=pipe, sort_every=7)
+    rh, vh, act = dev.track_batch(r0, v0, fe, fb, 210, 5e-11, 30)
+    host = pa.Pusher(ion, alg, precision=prec, electrode_assembly=orc_e)
+    rh2, vh2, act2 = host.track_batch(r0, v0, fe, fb, 210, 5e-11, 30)
+    np.testing.assert_array_equal(act, act2)
+    np.testing.assert_array_equal(rh, rh2)
+    np.testing.assert_array_equal(vh, vh2)
+    assert 0.1 < 1 - act.mean() < 0.95
+    if prec == "fp64":
+        orc = OraclePusher(ion.q_over_m, alg)
+        orh, ovh, oact = orc.track_batch(r0, v0, OField.from_arrays(grid, E), OField.from_arrays(grid, B), 210,
+                                         5e-11, 30, electrode_assembly=orc_e)
+        np.testing.assert_array_equal(act, oact)
+        np.testing.assert_array_equal(rh, orh)
+        np.testing.assert_array_equal(vh, ovh)
