@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--sort-every", type=int, default=500, help="steps between cell sorts (0: never)")
     ap.add_argument("--refine-bits", type=int, default=0, help="sub-cell sort bins along the motion (2^bits)")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: library default 256)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lossy"])
     ap.add_argument("--bunches", type=int, default=512)
     ap.add_argument("--c4-scale", type=float, default=1.0, help="scale C4 bunch sizes (1.0 = the C4 config)")
     ap.add_argument("--autotune", action="store_true")
@@ -153,39 +153,46 @@ def collision_rate(st_old, st_new, reps=5):
             "hits": int(hit.to_host().sum())}
 
 
-def lossy_tracking(w, fp, qm, nsteps=2000, radius=3e-3, precision="fp32_mixed"):
-    """The C2 bunch inside a beam pipe along its whole path: the loss test fused into the step loop
-    (pp_push_lossy), with compaction of the live slots every 500 steps (partition + cell sort), and
-    without compaction; against the same steps with no mesh (hot kernel)."""
+def lossy_tracking(w, fp, qm, nsteps=2000, radii=(3e-3, 10e-3), precision="fp32_mixed"):
+    """The C2 bunch inside a beam pipe along its whole path (64 x 256 quads): the loss test fused into
+    the step loop (pp_push_lossy) with compaction of the live slots every 500 steps (partition + cell
+    sort), and without compaction; against the same steps with no mesh, through the generic kernel
+    with an alive mask and through the hot kernel.  A 3 mm pipe loses about 28 % of the bunch (sigma
+    2 mm); a 10 mm pipe holds the bunch (the common case: the test runs, nothing is near a wall)."""
     from pypatools_amd.device import DeviceArray, synchronize
     from pypatools_amd.electrodes import TriangleElectrodes
     from pypatools_amd.engine import ParticleState, Tracker
     dt = w["dt"]
-    length = float(np.max(w["v0"][:, 2])) * nsteps * dt + 0.2
-    pipe = TriangleElectrodes.cylinder(radius, 0.0, length, n_phi=64, n_z=256)
     M = len(w["r0"])
-    res = {"particles": M, "steps": nsteps, "pipe_radius_m": radius, "pipe_length_m": length,
-           "triangles": pipe.n_triangles, "precision": precision}
-    for sort_every in (500, 0):
+    length = float(np.max(w["v0"][:, 2])) * nsteps * dt + 0.2
+
+    def run(mesh, sort_every, alive=True):
         st = ParticleState.from_host(w["r0"], w["v0"])
-        st.alive = DeviceArray.from_host(np.ones(M, dtype=np.uint8))
+        if alive:
+            st.alive = DeviceArray.from_host(np.ones(M, dtype=np.uint8))
         tr = Tracker("boris", precision, qm, sort_every=sort_every)
         active = np.ones(M, dtype=bool)
         synchronize()
         t = time.perf_counter()
-        tr._lossy_loop(st, fp, nsteps, dt, None, None, pipe, active, None, False)
+        if mesh is not None:
+            tr._lossy_loop(st, fp, nsteps, dt, None, None, mesh, active, None, False)
+        else:
+            tr._fused_loop(st, fp, nsteps, dt, 0, None, None, None)
         synchronize()
         el = time.perf_counter() - t
-        key = "compacted" if sort_every else "not_compacted"
-        res[key] = {"s": el, "particle_steps_per_s": M * nsteps / el, "lost_fraction": float(1 - active.mean())}
-    st = ParticleState.from_host(w["r0"], w["v0"])
-    tr = Tracker("boris", precision, qm)
-    synchronize()
-    t = time.perf_counter()
-    tr.run_resident(st, fp, nsteps, dt)
-    synchronize()
-    el = time.perf_counter() - t
-    res["no_mesh"] = {"s": el, "particle_steps_per_s": M * nsteps / el}
+        out = {"s": el, "particle_steps_per_s": M * nsteps / el}
+        if mesh is not None:
+            out["lost_fraction"] = float(1 - active.mean())
+        return out
+
+    res = {"particles": M, "steps": nsteps, "precision": precision, "pipe_length_m": length,
+           "no_mesh_generic_kernel": run(None, 500), "no_mesh_hot_kernel": run(None, 500, alive=False), "pipes": []}
+    for radius in radii:
+        pipe = TriangleElectrodes.cylinder(radius, 0.0, length, n_phi=64, n_z=256)
+        e = {"pipe_radius_m": radius, "triangles": pipe.n_triangles, "compacted": run(pipe, 500)}
+        if radius == radii[0]:
+            e["not_compacted"] = run(pipe, 0)
+        res["pipes"].append(e)
     return res
 
 
@@ -434,6 +441,31 @@ def bench_c5(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_lossy(args):
+    """C2 bunch and map with an electrode mesh: track_batch's loss test on the device (extension row)."""
+    import pypatools_amd as pa
+    from pypatools_amd.engine import FieldPair
+    from pypatools_amd.workloads import c2
+    w = c2(M=args.particles)
+    fp = FieldPair(pa.Field.from_arrays(w["grid"], w["E"], periodic_z=True),
+                   pa.Field.from_arrays(w["grid"], w["B"], periodic_z=True))
+    qm = pa.IonSpecies("proton").q_over_m
+    t0 = time.perf_counter()
+    res = lossy_tracking(w, fp, qm, precision=args.precision)
+    el = time.perf_counter() - t0
+    main_pipe = res["pipes"][0]["compacted"]
+    out = {"metric": "particle-steps/s of Boris tracking with the electrode loss test (C2 bunch in a beam pipe)",
+           "value": main_pipe["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": 1,
+           "warmup": 0, "ms_per_step": 1e3 * main_pipe["s"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64" if args.precision == "fp32_mixed" else "f64",
+           "data": "synthetic",
+           "config": {"workload": f"C2 bunch ({res['particles']} protons) and 128^3 map, {res['steps']} steps, "
+                                  "cylindrical pipe meshes of 32768 triangles along the whole path; fused loss test, "
+                                  "compaction every 500 steps", "precision": args.precision},
+           "extra": dict(res, wall_s=el)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -451,6 +483,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "lossy":
+        return bench_lossy(args)
 
     import pypatools_amd as pa
     from pypatools_amd.device import DeviceArray, Event, synchronize
