@@ -196,11 +196,25 @@ def lossy_tracking(w, fp, qm, nsteps=2000, radii=(3e-3, 10e-3), precision="fp32_
     return res
 
 
-def space_charge_timing(state, cells=(128, 128, 128), extent=(0.128, 0.128, 0.128), reps=3):
-    """Device space-charge step of the resident bunch: CIC deposit, direct Poisson solve, E = -grad(phi)."""
+def space_charge_timing(state, cells=(128, 128, 128), extent=(0.128, 0.128, 0.128), reps=3, electrodes=False):
+    """Device space-charge step of the resident bunch: CIC deposit, Poisson solve, E = -grad(phi).  Without
+    electrodes the solve is direct (sine transforms); with electrodes (four grounded rods of 10 x 10 mm at
+    30 mm from the axis, along the whole domain) it is BiCGSTAB on the Shortley-Weller system to a relative
+    residual of 1e-10, preconditioned by the direct box solve."""
     from pypatools_amd.device import Event, synchronize
+    from pypatools_amd.electrodes import TriangleElectrodes, box_mesh
     from pypatools_amd.space_charge import SpaceChargeSolver
-    s = SpaceChargeSolver(extent, cells)
+    assy = None
+    t_setup = 0.0
+    if electrodes:
+        rods = [box_mesh([cx - 0.005, cy - 0.005, -1.0], [cx + 0.005, cy + 0.005, 1.0])
+                for cx, cy in ((0.03, 0.0), (-0.03, 0.0), (0.0, 0.03), (0.0, -0.03))]
+        v = np.concatenate([r[0] for r in rods])
+        t = np.concatenate([r[1] + 8 * k for k, r in enumerate(rods)])
+        assy = TriangleElectrodes(v, t)
+    t0 = time.perf_counter()
+    s = SpaceChargeSolver(extent, cells, electrode_assembly=assy)
+    t_setup = time.perf_counter() - t0
     q = 1e-9 / state.n                                  # a 1 nC bunch
     s._solve_device(state.x.ptr, state.y.ptr, state.z.ptr, None, q, state.n)
     e0, e1 = Event(), Event()
@@ -211,7 +225,12 @@ def space_charge_timing(state, cells=(128, 128, 128), extent=(0.128, 0.128, 0.12
     synchronize()
     ms = e0.elapsed_ms(e1) / reps
     phi = s.phi.to_host()
-    return {"particles": state.n, "mesh": list(cells), "ms_per_solve": ms, "phi_max_V": float(phi.max())}
+    out = {"particles": state.n, "mesh": list(cells), "ms_per_solve": ms, "phi_max_V": float(phi.max())}
+    if electrodes:
+        it, res = s.solve_stats()
+        out.update({"electrodes": "4 grounded rods, 48 triangles", "setup_s": t_setup, "iterations": it,
+                    "relative_residual": res, "conductor_nodes": int((s.cell_type == 2).sum())})
+    return out
 
 
 def cpu_baseline(w, seconds=15.0):
@@ -620,6 +639,7 @@ def main():
     out["extra"]["electrode_collision"] = collision_rate(st0, st)
     out["extra"]["lossy_tracking"] = lossy_tracking(w, fp, qm)
     out["extra"]["space_charge"] = space_charge_timing(st0)
+    out["extra"]["space_charge_electrodes"] = space_charge_timing(st0, electrodes=True)
     out["config"]["sort_every"] = args.sort_every
     out["config"]["refine_bits"] = args.refine_bits
     if not args.no_cpu:

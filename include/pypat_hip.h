@@ -380,6 +380,34 @@ int pp_poisson_solve(const pp_poisson* p, const double* rho, double* phi, pp_str
 int pp_efield_from_potential(const pp_poisson* p, const double* phi, double* ex, double* ey, double* ez,
                              pp_stream s);
 
+/* Electrode-bounded domains (PyAMGPoissonSolver with an electrode assembly,
+   poisson_amg.py:241-444, 616-735).  pp_poisson_set_conductors takes host
+   vertices (nv, 3) and vertex-index triangles (nt, 3) of closed surfaces (solid,
+   grounded electrodes; nt == 0 removes them).  Along the six axis rays of every
+   node it counts surface crossings and measures the distance to the nearest one,
+   which stands in for py_electrodes' compute_axis_aligned_surface_intersections:
+   projected edge functions with a top-left rule count each crossing of a closed
+   surface once.  Nodes with an odd count on any ray are CONDUCTOR (2); others with
+   a distance at most the axis spacing (rounded to float32) are BOUNDARY (1), the
+   rest INTERIOR (0).  Non-conductor nodes get the reference's Shortley-Weller
+   rows (arm max(distance, 1e-3 h) towards a conductor neighbour).  From then on
+   pp_poisson_solve solves that system with phi = 0 on conductors, by BiCGSTAB
+   preconditioned with the box solve, to a relative residual rtol (default 1e-10,
+   at most 500 iterations; pp_poisson_set_tolerance); it synchronises the stream
+   once per iteration.  pp_poisson_conductor_data copies the cell types (N),
+   distances (N, 6: +x, -x, +y, -y, +z, -z; inf without a crossing) and crossing
+   counts (N, 6) to host arrays (each may be NULL).  pp_poisson_solve_stats
+   reports the iterations and relative residual of the last solve. */
+int pp_poisson_set_conductors(pp_poisson* p, const double* vertices, int64_t nv, const int32_t* triangles,
+                              int64_t nt);
+/* The same from ray data a caller computed (the arrays py_electrodes'
+   compute_axis_aligned_surface_intersections returns, poisson_amg.py:250-251):
+   host distances (N, 6) and crossing counts (N, 6), node order (nx, ny, nz). */
+int pp_poisson_set_ray_data(pp_poisson* p, const double* distances, const int32_t* counts);
+int pp_poisson_conductor_data(const pp_poisson* p, int32_t* cell_type, double* distances, int32_t* counts);
+int pp_poisson_set_tolerance(pp_poisson* p, double rtol, int32_t max_iterations);
+int pp_poisson_solve_stats(const pp_poisson* p, int32_t* iterations, double* relative_residual);
+
 /* Gather-bandwidth probe (measurement utility; no counterpart in the reference).
    nthreads threads each read `iters` cells of 192 B from the FP32 coefficient table
    of `map` through the same buffer-load path as the FP32 hot kernels and fold them

@@ -155,6 +155,11 @@ SIGNATURES = {
     "pp_cic_deposit": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double, c_void_p, c_int64,
                                  c_void_p, c_void_p]),
     "pp_poisson_solve": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pp_poisson_set_conductors": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int64]),
+    "pp_poisson_set_ray_data": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "pp_poisson_conductor_data": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pp_poisson_set_tolerance": (c_int32, [c_void_p, ctypes.c_double, c_int32]),
+    "pp_poisson_solve_stats": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(ctypes.c_double)]),
     "pp_efield_from_potential": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pp_mesh_create": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, ctypes.POINTER(c_void_p)]),
     "pp_mesh_destroy": (c_int32, [c_void_p]),

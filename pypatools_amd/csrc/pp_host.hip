@@ -961,6 +961,57 @@ int pp_poisson_solve(const pp_poisson* p, const double* rho, double* phi, pp_str
     return PP_OK;
 }
 
+int pp_poisson_set_conductors(pp_poisson* p, const double* vertices, int64_t nv, const int32_t* triangles,
+                              int64_t nt) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (nv < 0 || nt < 0 || nt > ((int64_t)1 << 30)) return fail(PP_ERR_INVALID, "vertex / triangle counts out of range");
+    if ((nv > 0 && !vertices) || (nt > 0 && !triangles)) return fail(PP_ERR_INVALID, "null host array");
+    char err[256] = {0};
+    rc = pp_poisson_impl::set_conductors(p, vertices, nv, triangles, nt, err, sizeof(err));
+    if (rc) return fail(rc, "%s", err);
+    return PP_OK;
+}
+
+int pp_poisson_set_ray_data(pp_poisson* p, const double* distances, const int32_t* counts) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (!distances || !counts) return fail(PP_ERR_INVALID, "null host array");
+    char err[256] = {0};
+    rc = pp_poisson_impl::set_ray_data(p, distances, counts, err, sizeof(err));
+    if (rc) return fail(rc, "%s", err);
+    return PP_OK;
+}
+
+int pp_poisson_conductor_data(const pp_poisson* p, int32_t* cell_type, double* distances, int32_t* counts) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (!pp_poisson_impl::has_conductors(p)) return fail(PP_ERR_INVALID, "no conductors are set");
+    pp_poisson_impl::conductor_data(p, cell_type, distances, counts);
+    return PP_OK;
+}
+
+int pp_poisson_set_tolerance(pp_poisson* p, double rtol, int32_t max_iterations) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    if (!(rtol > 0.0) || !std::isfinite(rtol) || max_iterations < 1)
+        return fail(PP_ERR_INVALID, "rtol must be positive and max_iterations at least 1");
+    pp_poisson_impl::set_tolerance(p, rtol, max_iterations);
+    return PP_OK;
+}
+
+int pp_poisson_solve_stats(const pp_poisson* p, int32_t* iterations, double* relative_residual) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if ((rc = poisson_check(p))) return rc;
+    pp_poisson_impl::stats(p, iterations, relative_residual);
+    return PP_OK;
+}
+
 int pp_efield_from_potential(const pp_poisson* p, const double* phi, double* ex, double* ey, double* ez,
                              pp_stream s) {
     int rc = ensure_init();

@@ -198,6 +198,14 @@ hipError_t deposit(const pp_poisson* p, const double* x, const double* y, const 
                    double qc, const uint8_t* alive, int64_t n, double* rho, hipStream_t s);
 hipError_t solve(const pp_poisson* p, const double* rho, double* phi, hipStream_t s);
 hipError_t gradient(const pp_poisson* p, const double* phi, double* ex, double* ey, double* ez, hipStream_t s);
+// electrode-bounded domains: classification and Shortley-Weller operator from a closed triangle mesh
+int set_conductors(pp_poisson* p, const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, char* err,
+                   size_t errn);
+int set_ray_data(pp_poisson* p, const double* dist, const int32_t* count, char* err, size_t errn);
+bool has_conductors(const pp_poisson* p);
+void conductor_data(const pp_poisson* p, int32_t* type, double* dist, int32_t* count);
+void set_tolerance(pp_poisson* p, double rtol, int32_t max_iter);
+void stats(const pp_poisson* p, int32_t* iterations, double* residual);
 }  // namespace pp_poisson_impl
 
 struct pp_mesh;

@@ -73,3 +73,108 @@ def test_validation():
         SpaceChargeSolver((0.01, 0.01, 0.01), (2, 8, 8))
     with pytest.raises(ValueError):
         SpaceChargeSolver((0.01, -0.01, 0.01), (8, 8, 8))
+
+
+# --------------------------------------------------------------------------- electrode-bounded domains
+
+from pypatools_amd.electrodes import TriangleElectrodes, box_mesh, sphere_mesh      # noqa: E402
+
+
+def _merge(*meshes):
+    vs, ts, off = [], [], 0
+    for v, t in meshes:
+        vs.append(v)
+        ts.append(t + off)
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ts)
+
+
+def _meshes():
+    return {
+        "half_space": (((0.1, 0.08, 0.06), (10, 8, 6)), box_mesh([-1.0, -1.0, -1.0], [0.0123, 1.0, 1.0])),
+        "cube_diagonals": (((0.08, 0.08, 0.08), (8, 8, 8)), box_mesh([-0.023] * 3, [0.023] * 3)),
+        "sphere_and_slab": (((0.1, 0.1, 0.1), (18, 16, 14)),
+                            _merge(sphere_mesh([0.012, -0.004, 0.003], 0.021, 2),
+                                   box_mesh([-1.0, -1.0, -0.0371], [1.0, 1.0, -0.0297]))),
+    }
+
+
+@pytest.mark.parametrize("name", ["half_space", "cube_diagonals", "sphere_and_slab"])
+def test_conductor_classification_matches_oracle(name):
+    """Cell types, axis-ray distances and crossing counts from the library equal the oracle's bit for bit."""
+    (ext, cells), (v, t) = _meshes()[name]
+    s = SpaceChargeSolver(ext, cells, electrode_assembly=TriangleElectrodes(v, t))
+    ct, dist, counts = s.conductor_data()
+    d_o, c_o = SC.axis_ray_queries(v, t, ext, cells)
+    np.testing.assert_array_equal(counts, c_o)
+    np.testing.assert_array_equal(dist, d_o)
+    np.testing.assert_array_equal(ct, SC.classify(ext, cells, d_o, c_o))
+    assert (ct == SC.CONDUCTOR).sum() > 0 and (ct != SC.CONDUCTOR).sum() > 0
+
+
+@pytest.mark.parametrize("name", ["half_space", "sphere_and_slab"])
+def test_electrode_bounded_potential_matches_oracle(name):
+    """Deposit a bunch between the electrodes, solve on the device (BiCGSTAB, box-solve preconditioner) and
+    compare with the sparse direct solve of the oracle's Shortley-Weller system; phi is 0 on conductors."""
+    (ext, cells), (v, t) = _meshes()[name]
+    rng = np.random.default_rng(7)
+    n = 5000
+    pos = np.column_stack([rng.uniform(0.02, 0.04, n), rng.uniform(-0.02, 0.02, n), rng.uniform(-0.02, 0.02, n)])
+    q = rng.uniform(1e-16, 3e-16, n)
+    s = SpaceChargeSolver(ext, cells, electrode_assembly=TriangleElectrodes(v, t), solver_tol=1e-12)
+    phi, E = s.solve(pos, q)
+    it, res = s.solve_stats()
+    assert 0 < it < 500 and res <= 1e-12
+    ct = s.cell_type
+    d_o, c_o = SC.axis_ray_queries(v, t, ext, cells)
+    rho_o = SC.cic_deposit(ext, cells, pos, q)
+    phi_o = SC.solve_with_conductors(ext, cells, rho_o, SC.classify(ext, cells, d_o, c_o), d_o)
+    np.testing.assert_allclose(phi, phi_o, rtol=0, atol=1e-9 * np.abs(phi_o).max())
+    assert np.all(phi.ravel()[ct == SC.CONDUCTOR] == 0.0)
+    g = s.grid()
+    Eo = SC.gradient(ext, cells, phi_o)
+    pts = np.column_stack([np.full(4, g["x"][5]), g["y"][[3, 4, 5, 6]], np.full(4, g["z"][4])])
+    np.testing.assert_allclose(E(pts)[:, 0], Eo[0][5, [3, 4, 5, 6], 4], rtol=1e-7, atol=1e-9 * np.abs(Eo[0]).max())
+
+
+def test_manufactured_solution_on_device_converges():
+    """The oracle's manufactured solution (zero on the electrode face x = c and one spacing outside the box)
+    solved on the device: second-order convergence of the Shortley-Weller system."""
+    ext, base = (0.1, 0.08, 0.06), (10, 8, 6)
+    c = 0.0123
+    v, t = box_mesh([-1.0, -1.0, -1.0], [c, 1.0, 1.0])
+    mesh = TriangleElectrodes(v, t)
+    errs = []
+    for k in (2, 4):
+        cells = tuple(n * k for n in base)
+        h = [L / n for L, n in zip(ext, cells)]
+        X, Y, Z = [L / 2 + hh / 2 for L, hh in zip(ext, h)]
+        s = SpaceChargeSolver(ext, cells, electrode_assembly=mesh)
+        P = s.mesh_nodes()
+        x, y, z = P[:, 0], P[:, 1], P[:, 2]
+        ex = np.sin(np.pi * (x - c) / (X - c)) * np.sin(np.pi * (y + Y) / (2 * Y)) * np.sin(np.pi * (z + Z) / (2 * Z))
+        lam = (np.pi / (X - c)) ** 2 + (np.pi / (2 * Y)) ** 2 + (np.pi / (2 * Z)) ** 2
+        phi = s.solve_density(SC.EPS0 * lam * ex).ravel()
+        act = s.cell_type != SC.CONDUCTOR
+        errs.append(np.max(np.abs(phi[act] - ex[act])) / np.max(np.abs(ex[act])))
+    assert errs[0] / errs[1] > 3.0, errs
+
+
+def test_py_electrodes_ray_protocol_gives_the_same_solution():
+    """An assembly that answers compute_axis_aligned_surface_intersections (the py_electrodes query of
+    poisson_amg.py:250-251) yields the same operator and potential as the mesh itself."""
+    (ext, cells), (v, t) = _meshes()["sphere_and_slab"]
+
+    class RayAssembly:
+        def compute_axis_aligned_surface_intersections(self, nodes, axes="all", use_gpu=False):
+            assert nodes.shape == (int(np.prod(cells)), 3)
+            return SC.axis_ray_queries(v, t, ext, cells)
+
+    rng = np.random.default_rng(8)
+    rho = np.abs(rng.standard_normal(cells)) * 1e-9
+    a = SpaceChargeSolver(ext, cells, electrode_assembly=TriangleElectrodes(v, t))
+    b = SpaceChargeSolver(ext, cells, electrode_assembly=RayAssembly())
+    np.testing.assert_array_equal(a.cell_type, b.cell_type)
+    np.testing.assert_array_equal(a.solve_density(rho), b.solve_density(rho))
+    with pytest.raises(TypeError):
+        SpaceChargeSolver(ext, cells, electrode_assembly=object())
