@@ -103,17 +103,22 @@ __global__ void __launch_bounds__(kDepositBlock) k_deposit(const PoissonDesc d, 
 }
 
 // Batched row-major FP64 GEMM, C[b] = alpha A[b] B[b], for the axis transforms (each one is a
-// product with the n x n sine matrix).  64 x 64 output tiles, 16-deep k slices staged in LDS,
-// 256 threads with a 4 x 4 register block each; edges are zero-filled.
+// product with the n x n sine matrix), on the FP64 matrix cores: BM x BN output tiles, BK-deep k slices
+// staged in LDS (edges zero-filled), four wavefronts (2 x 2) with 16 x 16 tiles of v_mfma_f64_16x16x4.
+// 32 x 32 x 32 tiles give products up to 128^3 enough workgroups to fill the chip; 64 x 64 x 16 serve larger ones.
+// Lane l of a 16x16x4 step holds A[l % 16][l / 16] and B[l / 16][l % 16]; its four results are
+// D[l / 16 + 4 r][l % 16], r = 0..3 (rows interleaved across the lane groups).
 struct Gemm {
     const double* A; const double* B; double* C;
     int64_t M, N, K, lda, ldb, ldc, sa, sb, sc;
     double alpha;
 };
 
-constexpr int kBM = 64, kBN = 64, kBK = 16;
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_gemm(const Gemm g) {
+template <int kBM, int kBN, int kBK>
+__global__ void __launch_bounds__(256) k_gemm_mfma(const Gemm g) {
+    constexpr int kTU = kBM / 32, kTV = kBN / 32;   // 16 x 16 MFMA tiles per wavefront
     __shared__ double As[kBK][kBM + 1];
     __shared__ double Bs[kBK][kBN + 1];
     const int64_t b = blockIdx.z;
@@ -121,8 +126,14 @@ __global__ void __launch_bounds__(256) k_gemm(const Gemm g) {
     const double* B = g.B + b * g.sb;
     double* C = g.C + b * g.sc;
     const int64_t m0 = (int64_t)blockIdx.y * kBM, n0 = (int64_t)blockIdx.x * kBN;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4] = {};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * (kBM / 2), wn = (wave & 1) * (kBN / 2);   // this wavefront's quarter of the tile
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t acc[kTU][kTV];
+#pragma unroll
+    for (int u = 0; u < kTU; ++u)
+#pragma unroll
+        for (int v = 0; v < kTV; ++v) acc[u][v] = double4_t{0.0, 0.0, 0.0, 0.0};
     for (int64_t k0 = 0; k0 < g.K; k0 += kBK) {
         for (int e = threadIdx.x; e < kBM * kBK; e += 256) {
             const int r = e / kBK, c = e % kBK;            // A tile: rows m0 + r, columns k0 + c
@@ -136,27 +147,28 @@ __global__ void __launch_bounds__(256) k_gemm(const Gemm g) {
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < kBK; ++kk) {
-            double av[4], bv[4];
+        for (int kk = 0; kk < kBK; kk += 4) {
+            double a[kTU], bb[kTV];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                av[u] = As[kk][ty + 16 * u];
-                bv[u] = Bs[kk][tx + 16 * u];
-            }
+            for (int u = 0; u < kTU; ++u) a[u] = As[kk + lk][wm + 16 * u + li];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int v = 0; v < kTV; ++v) bb[v] = Bs[kk + lk][wn + 16 * v + li];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+            for (int u = 0; u < kTU; ++u)
+#pragma unroll
+                for (int v = 0; v < kTV; ++v) acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[v], acc[u][v], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < kTU; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t gm = m0 + ty + 16 * u, gn = n0 + tx + 16 * v;
-            if (gm < g.M && gn < g.N) C[gm * g.ldc + gn] = g.alpha * acc[u][v];
-        }
+        for (int v = 0; v < kTV; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gm = m0 + wm + 16 * u + lk + 4 * r, gn = n0 + wn + 16 * v + li;
+                if (gm < g.M && gn < g.N) C[gm * g.ldc + gn] = g.alpha * acc[u][v][r];
+            }
 }
 
 // bhat / (lambda_x[i] + lambda_y[j] + lambda_z[k]) * scale, with b = rho / eps0 folded into scale
@@ -423,9 +435,14 @@ hipError_t deposit(const pp_poisson* P, const double* x, const double* y, const 
 }
 
 static void gemm(const pp_poissonk::Gemm& g, int64_t batch, hipStream_t s) {
-    const dim3 grid((unsigned)((g.N + pp_poissonk::kBN - 1) / pp_poissonk::kBN),
-                    (unsigned)((g.M + pp_poissonk::kBM - 1) / pp_poissonk::kBM), (unsigned)batch);
-    hipLaunchKernelGGL(pp_poissonk::k_gemm, grid, dim3(256), 0, s, g);
+    const int64_t tiles64 = ((g.N + 63) / 64) * ((g.M + 63) / 64) * batch;
+    if (tiles64 < 2048) {
+        const dim3 grid((unsigned)((g.N + 31) / 32), (unsigned)((g.M + 31) / 32), (unsigned)batch);
+        hipLaunchKernelGGL((pp_poissonk::k_gemm_mfma<32, 32, 32>), grid, dim3(256), 0, s, g);
+    } else {
+        const dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 63) / 64), (unsigned)batch);
+        hipLaunchKernelGGL((pp_poissonk::k_gemm_mfma<64, 64, 16>), grid, dim3(256), 0, s, g);
+    }
 }
 
 // applies S along every axis of `in` (nx, ny, nz) into `out`, via `tmp`; the last product is scaled
