@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--sort-every", type=int, default=500, help="steps between cell sorts (0: never)")
     ap.add_argument("--refine-bits", type=int, default=0, help="sub-cell sort bins along the motion (2^bits)")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: library default 256)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lossy"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lossy", "spacecharge"])
     ap.add_argument("--bunches", type=int, default=512)
     ap.add_argument("--c4-scale", type=float, default=1.0, help="scale C4 bunch sizes (1.0 = the C4 config)")
     ap.add_argument("--autotune", action="store_true")
@@ -485,6 +485,24 @@ def bench_lossy(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_spacecharge(args):
+    """Space-charge step of the C2 bunch (1e6 protons, 1 nC) on a 128^3 mesh, box domain and four grounded rods."""
+    from pypatools_amd.engine import ParticleState
+    from pypatools_amd.workloads import c2
+    w = c2(M=args.particles)
+    st = ParticleState.from_host(w["r0"] - np.array([0.0, 0.0, 0.0635]), w["v0"])     # bunch centred in the box
+    box = space_charge_timing(st, reps=5)
+    rods = space_charge_timing(st, reps=3, electrodes=True)
+    out = {"metric": "ms per space-charge step (CIC deposit + Poisson solve + E = -grad phi), C2 bunch, 128^3 mesh",
+           "value": box["ms_per_solve"], "unit": "ms", "n_gpus": 1, "steps": 5, "warmup": 1,
+           "ms_per_step": box["ms_per_solve"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"C2 bunch ({st.n} protons, 1 nC) centred in a 0.128 m box, 128^3 cell-centre mesh; "
+                                  "box domain (direct sine-transform solve) and four grounded rods (BiCGSTAB)"},
+           "extra": {"box": box, "four_rods": rods}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -504,6 +522,8 @@ def main():
         return bench_c5(args)
     if args.workload == "lossy":
         return bench_lossy(args)
+    if args.workload == "spacecharge":
+        return bench_spacecharge(args)
 
     import pypatools_amd as pa
     from pypatools_amd.device import DeviceArray, Event, synchronize
