@@ -360,13 +360,13 @@ DEV double cevald(double2 ab, double2 cd, double2 ef, double2 gh, double tx, dou
     return fma(tz, p4, r);
 }
 
-// cell index and weights on one non-periodic axis, u-space bounds test
-DEV uint32_t hot_axis(double x, double id, double c, int nc, double& t, bool& ok) {
+// cell index (an integer-valued double) and weight on one non-periodic axis, u-space bounds test
+DEV double hot_axis(double x, double id, double c, int nc, double& t, bool& ok) {
     const double u = fma(x, id, c);
     ok = ok && (u >= 0.0) && (u <= (double)nc);
     const double fu = fmin(fmax(floor(u), 0.0), (double)(nc - 1));
     t = fmin(fmax(u - fu, 0.0), 1.0);
-    return (uint32_t)fu;
+    return fu;
 }
 
 // periodic axis: k = floor(u) mod nc in integer arithmetic.  floor(u) is clamped
@@ -386,16 +386,20 @@ DEV uint32_t hot_axis_periodic(double z, double id, double c, uint32_t nc, uint3
     return k;
 }
 
+// The hot paths take coefficient tables below 2^31 bytes (192 or 384 per cell), so every cell index is
+// below 2^24: the index i nc1 nc2 + j nc2 + k is exact as an FP64 fma chain on the integer-valued axis
+// indices, and the byte offsets cell * 192 (or 384) are exact as 24-bit multiplies (full rate, where
+// 32-bit integer multiplies run at quarter rate).
 template <bool PERIODIC>
 DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& tx, double& ty, double& tz,
                         bool& ok) {
     ok = true;
-    const uint32_t i = hot_axis(x, m.id0, m.c0, m.nc0, tx, ok);
-    const uint32_t j = hot_axis(y, m.id1, m.c1, m.nc1, ty, ok);
-    uint32_t k;
-    if constexpr (PERIODIC) k = hot_axis_periodic(z, m.id2, m.c2, (uint32_t)m.nc2, m.zmagic, m.zbias, tz);
+    const double i = hot_axis(x, m.id0, m.c0, m.nc0, tx, ok);
+    const double j = hot_axis(y, m.id1, m.c1, m.nc1, ty, ok);
+    double k;
+    if constexpr (PERIODIC) k = (double)hot_axis_periodic(z, m.id2, m.c2, (uint32_t)m.nc2, m.zmagic, m.zbias, tz);
     else k = hot_axis(z, m.id2, m.c2, m.nc2, tz, ok);
-    return (i * (uint32_t)m.nc1 + j) * (uint32_t)m.nc2 + k;
+    return (uint32_t)fma(i, m.nc12d, fma(j, m.nc2d, k));
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -489,9 +493,10 @@ struct HotFE32 {
             }
             if (todo == 0) {
                 float4* wv = stage[threadIdx.x >> 6];
-                if (lslot < 5u && lslot < (uint32_t)ns) wv[lslot * kStagePitch + lpart] = ld4f(rs, ld0 * 192u + lpart * 16u);
+                if (lslot < 5u && lslot < (uint32_t)ns)
+                    wv[lslot * kStagePitch + lpart] = ld4f(rs, __umul24(ld0, 192u) + lpart * 16u);
                 if (lslot < 5u && lslot + 5u < (uint32_t)ns)
-                    wv[(lslot + 5u) * kStagePitch + lpart] = ld4f(rs, ld1 * 192u + lpart * 16u);
+                    wv[(lslot + 5u) * kStagePitch + lpart] = ld4f(rs, __umul24(ld1, 192u) + lpart * 16u);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -504,7 +509,7 @@ struct HotFE32 {
         }
 #endif
         if (!done) {
-            const uint32_t base = cell * (48u * 4u);
+            const uint32_t base = __umul24(cell, 48u * 4u);
 #pragma unroll
             for (int c = 0; c < 6; ++c)
                 v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
@@ -538,7 +543,7 @@ struct HotFE64 {
         double tx, ty, tz;
         bool ok;
         const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
-        const uint32_t base = cell * (48u * 8u);
+        const uint32_t base = __umul24(cell, 48u * 8u);
         double v[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {

@@ -393,7 +393,8 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
     h.c0 = -m.lo[0] * m.inv_d[0];
     h.c1 = -m.lo[1] * m.inv_d[1];
     h.c2 = -m.lo[2] * m.inv_d[2];
-    h.inv_nz = m.inv_nz;
+    h.nc2d = (double)m.nc[2];
+    h.nc12d = (double)m.nc[1] * (double)m.nc[2];
     if (m.nc[2] > 0) {
         h.zmagic = (uint32_t)(0xFFFFFFFFull / (uint64_t)m.nc[2]);
         h.zbias = (uint32_t)m.nc[2] * ((1u << 30) / (uint32_t)m.nc[2]);

@@ -74,7 +74,7 @@ struct HotMap {
     int32_t nc0, nc1, nc2;  // cells per axis
     double c0, c1, c2;      // -lo[a] * inv_d[a]
     double id0, id1, id2;   // inv_d[a]
-    double inv_nz;          // 1 / nc2 (periodic z)
+    double nc2d, nc12d;     // nc2 and nc1 * nc2 as doubles: cell = i nc12d + j nc2d + k, exact in FP64
     double sE, sB;          // scale1 * scale2 of E and B
     double fill;            // value outside the grid (Field: 0)
     uint32_t zmagic;        // floor((2^32 - 1) / nc2): quotient estimate for the periodic z wrap
