@@ -445,12 +445,18 @@ DEV double hot_rf(const HotRF& rf, int mode, int64_t gs, int stage) {
     return rf.table[(gs - rf.step0) * 8 + stage * 2 + (mode == PP_RF_SIN ? 1 : 0)];
 }
 
-template <bool PERIODIC, bool RF = false>
+// REUSE (the RK4 kernels, whose four stages and successive steps mostly stay in the same cells): the
+// wavefront remembers the cells it staged through LDS last; while every lane is still in its cell, the
+// coefficients are read from the same slots without a new staging (same values, same result).
+template <bool PERIODIC, bool RF = false, bool REUSE = false>
 struct HotFE32 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
     HotScale k;
     HotRF rf;
+    mutable uint32_t last_cell = 0;   // REUSE: this lane's cell and LDS slot at the last staging
+    mutable uint32_t last_slot = 0;
+    mutable bool staged = false;      // REUSE: the wavefront has staged at least once (wave-uniform)
     DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B) const {
         double tx, ty, tz;
         bool ok;
@@ -459,6 +465,15 @@ struct HotFE32 {
         float v[6];
         bool done = false;
 #if defined(__HIP_DEVICE_COMPILE__)
+        __shared__ float4 stage_lds[kStageWaves][kStageSlots * kStagePitch];
+        if constexpr (REUSE) {
+            if (staged && __builtin_amdgcn_ballot_w64(cell != last_cell) == 0) {
+                const float4* q = stage_lds[threadIdx.x >> 6] + last_slot * kStagePitch;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
+                done = true;
+            }
+        }
         // Three ways to the cell coefficients, chosen per wavefront:
         //  - all lanes in one cell (common right after a cell sort): the cell's
         //    192 B are read once through the scalar cache;
@@ -468,13 +483,13 @@ struct HotFE32 {
         //  - otherwise every lane gathers its cell through the vector memory path.
         // All three evaluate the same FMA chain on the same values.
         const uint32_t c0 = __builtin_amdgcn_readfirstlane(cell);
-        if (__builtin_amdgcn_ballot_w64(cell != c0) == 0) {
+        if (done) {
+        } else if (__builtin_amdgcn_ballot_w64(cell != c0) == 0) {
             const cfloat_p q = (cfloat_p)m.coef + (size_t)c0 * 48u;
 #pragma unroll
             for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, w);
             done = true;
         } else if (__builtin_amdgcn_ballot_w64(true) == ~0ull) {
-            __shared__ float4 stage[kStageWaves][kStageSlots * kStagePitch];
             const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
             const uint32_t lslot = lane / 12u, lpart = lane - 12u * lslot;   // staging role of this lane
             uint64_t todo = ~0ull;
@@ -492,7 +507,7 @@ struct HotFE32 {
                 ns = it + 1;
             }
             if (todo == 0) {
-                float4* wv = stage[threadIdx.x >> 6];
+                float4* wv = stage_lds[threadIdx.x >> 6];
                 if (lslot < 5u && lslot < (uint32_t)ns)
                     wv[lslot * kStagePitch + lpart] = ld4f(rs, __umul24(ld0, 192u) + lpart * 16u);
                 if (lslot < 5u && lslot + 5u < (uint32_t)ns)
@@ -505,6 +520,11 @@ struct HotFE32 {
                 for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
                 __builtin_amdgcn_wave_barrier();
                 done = true;
+                if constexpr (REUSE) {
+                    last_cell = cell;
+                    last_slot = myslot;
+                    staged = true;
+                }
             }
         }
 #endif
@@ -580,6 +600,8 @@ constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || AL
 
 // 4 wavefronts per SIMD (<= 128 VGPRs) keeps enough loads and LDS reads in flight;
 // the compiler would otherwise spend registers for a deeper load pipeline at 3.
+// Four wavefronts per SIMD also for the RK4 steps: measured on C3, the spills this costs are cheaper than
+// halving the occupancy (2 wavefronts per SIMD ran 1.6x slower without staging reuse).
 template <int ALG, bool MIXED, bool PERIODIC, bool RF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -604,7 +626,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     const float h = (float)(0.5 * p.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
-    const HotFE32<PERIODIC, RF> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0), p.rf};
+    constexpr bool REUSE = ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL;
+    const HotFE32<PERIODIC, RF, REUSE> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0), p.rf};
     const bool rec_on = p.r_hist != nullptr;
     int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
     int64_t rec_idx = p.step0 / p.rec_every + 1;
