@@ -661,7 +661,7 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
     d3 vrec = v;
     const float h = (float)(0.5 * ac.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
-    const HotFE32<PERIODIC> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0), HotRF{}};
+    const HotFE32<PERIODIC, false, ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0), HotRF{}};
     for (int64_t s = 0; s < ns; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs, HotRF{}}, ac);
