@@ -157,3 +157,41 @@ def test_lossy_track_through_a_map_with_compaction(alg, prec):
         np.testing.assert_array_equal(act, oact)
         np.testing.assert_array_equal(rh, orh)
         np.testing.assert_array_equal(vh, ovh)
+
+
+def test_every_particle_lost_ends_the_run_early():
+    """All particles leave a small closed box within a few steps: the device path stops launching when no
+    slot is live; histories after each loss stay NaN and equal the host-callback route and the oracle."""
+    rng = np.random.default_rng(54)
+    n = 3000
+    box = TriangleElectrodes.box([-1e-3, -1e-3, -1e-3], [1e-3, 1e-3, 1e-3])
+    orc_e = OracleElectrodes(box.vertices, box.triangles)
+    r0 = rng.uniform(-5e-4, 5e-4, (n, 3))
+    v0 = rng.standard_normal((n, 3)) * 2e6
+    E, B = pa.Field(dim=0, field={"x": 0.0, "y": 0.0, "z": 0.0}), pa.Field(dim=0, field={"x": 0.0, "y": 0.0, "z": 0.1})
+    ion = pa.IonSpecies("proton")
+    rh, vh, act = pa.Pusher(ion, "boris", electrode_assembly=box, sort_every=3).track_batch(r0, v0, E, B, 200, 1e-10, 5)
+    rh2, vh2, act2 = pa.Pusher(ion, "boris", electrode_assembly=orc_e).track_batch(r0, v0, E, B, 200, 1e-10, 5)
+    assert not act.any()
+    np.testing.assert_array_equal(act, act2)
+    np.testing.assert_array_equal(rh, rh2)
+    np.testing.assert_array_equal(vh, vh2)
+    orh, ovh, oact = OraclePusher(ion.q_over_m, "boris").track_batch(
+        r0, v0, OField.constant(0, 0, 0), OField.constant(0, 0, 0.1), 200, 1e-10, 5, electrode_assembly=orc_e)
+    np.testing.assert_array_equal(rh, orh)
+    assert np.all(np.isnan(rh[-1]))
+
+
+def test_merged_electrodes_collide_like_their_parts():
+    """A merged assembly (box + tube) hits exactly where either part does."""
+    rng = np.random.default_rng(55)
+    a = TriangleElectrodes.box([-0.3, -0.3, 0.2], [0.3, 0.3, 0.5])
+    b = TriangleElectrodes.cylinder(0.8, -1.0, 1.0, n_phi=32, n_z=8)
+    m = TriangleElectrodes.merge(a, b)
+    r0 = rng.uniform(-1, 1, (4000, 3))
+    r1 = r0 + 0.4 * rng.standard_normal((4000, 3))
+    hm = m.segment_intersects_surface(r0, r1)["hit_mask"]
+    ha = a.segment_intersects_surface(r0, r1)["hit_mask"]
+    hb = b.segment_intersects_surface(r0, r1)["hit_mask"]
+    np.testing.assert_array_equal(hm, ha | hb)
+    assert ha.sum() > 0 and hb.sum() > 0
