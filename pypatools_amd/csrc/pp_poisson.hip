@@ -424,8 +424,10 @@ int device_of(const pp_poisson* P) { return P->device; }
 
 hipError_t deposit(const pp_poisson* P, const double* x, const double* y, const double* z, const double* q,
                    double qc, const uint8_t* alive, int64_t n, double* rho, hipStream_t s) {
-    // about 2048 blocks, each a contiguous chunk of at least 16 particles per thread
-    const int64_t per = (int64_t)pp_poissonk::kDepositBlock * 16;
+    // about 2048 blocks, each a contiguous chunk of at least 4 particles per thread: 1e6 particles make
+    // 245 blocks, about one per CU (16 per thread left most CUs idle; 1 per thread flushes the LDS table
+    // too often)
+    const int64_t per = (int64_t)pp_poissonk::kDepositBlock * 4;
     int64_t chunk = (n + 2047) / 2048;
     chunk = chunk < per ? per : chunk;
     const unsigned nb = (unsigned)((n + chunk - 1) / chunk);
