@@ -118,9 +118,13 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 template <int kBM, int kBN, int kBK>
 __global__ void __launch_bounds__(256) k_gemm_mfma(const Gemm g) {
-    constexpr int kTU = kBM / 32, kTV = kBN / 32;   // 16 x 16 MFMA tiles per wavefront
-    __shared__ double As[kBK][kBM + 1];
-    __shared__ double Bs[kBK][kBN + 1];
+    constexpr int kTU = kBM / 32, kTV = kBN / 32;             // 16 x 16 MFMA tiles per wavefront
+    constexpr int kLA = kBM * kBK / 256, kLB = kBK * kBN / 256;   // tile elements each thread moves per k slice
+    static_assert(kLA * 256 == kBM * kBK && kLB * 256 == kBK * kBN, "tile must split evenly over 256 threads");
+    // two LDS buffers: the next k slice is read from global memory into registers while the matrix cores work
+    // on the current one, then written to the other buffer (one barrier per slice)
+    __shared__ double As[2][kBK][kBM + 1];
+    __shared__ double Bs[2][kBK][kBN + 1];
     const int64_t b = blockIdx.z;
     const double* A = g.A + b * g.sa;
     const double* B = g.B + b * g.sb;
@@ -134,31 +138,55 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const Gemm g) {
     for (int u = 0; u < kTU; ++u)
 #pragma unroll
         for (int v = 0; v < kTV; ++v) acc[u][v] = double4_t{0.0, 0.0, 0.0, 0.0};
-    for (int64_t k0 = 0; k0 < g.K; k0 += kBK) {
-        for (int e = threadIdx.x; e < kBM * kBK; e += 256) {
-            const int r = e / kBK, c = e % kBK;            // A tile: rows m0 + r, columns k0 + c
+    double ra[kLA], rb[kLB];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < kLA; ++q) {
+            const int e = threadIdx.x + 256 * q, r = e / kBK, c = e % kBK;   // A tile: rows m0 + r, columns k0 + c
             const int64_t gm = m0 + r, gk = k0 + c;
-            As[c][r] = (gm < g.M && gk < g.K) ? A[gm * g.lda + gk] : 0.0;
+            ra[q] = (gm < g.M && gk < g.K) ? A[gm * g.lda + gk] : 0.0;
         }
-        for (int e = threadIdx.x; e < kBK * kBN; e += 256) {
-            const int r = e / kBN, c = e % kBN;            // B tile: rows k0 + r, columns n0 + c
+#pragma unroll
+        for (int q = 0; q < kLB; ++q) {
+            const int e = threadIdx.x + 256 * q, r = e / kBN, c = e % kBN;   // B tile: rows k0 + r, columns n0 + c
             const int64_t gk = k0 + r, gn = n0 + c;
-            Bs[r][c] = (gk < g.K && gn < g.N) ? B[gk * g.ldb + gn] : 0.0;
+            rb[q] = (gk < g.K && gn < g.N) ? B[gk * g.ldb + gn] : 0.0;
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < kLA; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            As[buf][e % kBK][e / kBK] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < kLB; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            Bs[buf][e / kBN][e % kBN] = rb[q];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = 0; k0 < g.K; k0 += kBK) {
+        const bool more = k0 + kBK < g.K;
+        if (more) load(k0 + kBK);
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 4) {
             double a[kTU], bb[kTV];
 #pragma unroll
-            for (int u = 0; u < kTU; ++u) a[u] = As[kk + lk][wm + 16 * u + li];
+            for (int u = 0; u < kTU; ++u) a[u] = As[buf][kk + lk][wm + 16 * u + li];
 #pragma unroll
-            for (int v = 0; v < kTV; ++v) bb[v] = Bs[kk + lk][wn + 16 * v + li];
+            for (int v = 0; v < kTV; ++v) bb[v] = Bs[buf][kk + lk][wn + 16 * v + li];
 #pragma unroll
             for (int u = 0; u < kTU; ++u)
 #pragma unroll
                 for (int v = 0; v < kTV; ++v) acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[v], acc[u][v], 0, 0, 0);
         }
+        if (more) store(buf ^ 1);   // the other buffer's readers finished before the previous barrier
         __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int u = 0; u < kTU; ++u)
