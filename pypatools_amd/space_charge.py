@@ -19,6 +19,7 @@ fields of a push.
   the reference asks it.
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -47,6 +48,7 @@ class SpaceChargeSolver:
         self.E = [DeviceArray((n,)) for _ in range(3)]
         self.turn_count = 0
         self.electrodes = electrode_assembly
+        self.solver_tol = float(solver_tol)
         _lib.call("pp_poisson_set_tolerance", self.handle, float(solver_tol), int(max_iterations))
         if electrode_assembly is not None:
             self._set_electrodes(electrode_assembly)
@@ -120,6 +122,11 @@ class SpaceChargeSolver:
         _lib.call("pp_poisson_solve", self.handle, self.rho.ptr, self.phi.ptr, None)
         _lib.call("pp_efield_from_potential", self.handle, self.phi.ptr, *(e.ptr for e in self.E), None)
         self.turn_count += 1
+        if self.electrodes is not None:
+            it, res = self.solve_stats()
+            if not res <= self.solver_tol:      # poisson_amg.py:711-715 warns when GMRES stops short
+                warnings.warn(f"BiCGSTAB did not converge: relative residual {res:.2e} after {it} iterations "
+                              f"(tolerance {self.solver_tol:.1e})", RuntimeWarning)
 
     def solve_density(self, rho):
         """phi (nx, ny, nz) for a given charge density rho [C/m^3] on the nodes (and the field in self.E)."""

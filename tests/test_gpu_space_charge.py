@@ -178,3 +178,15 @@ def test_py_electrodes_ray_protocol_gives_the_same_solution():
     np.testing.assert_array_equal(a.solve_density(rho), b.solve_density(rho))
     with pytest.raises(TypeError):
         SpaceChargeSolver(ext, cells, electrode_assembly=object())
+
+
+def test_unconverged_electrode_solve_warns():
+    """Like the reference's GMRES warning (poisson_amg.py:711-715): an iteration cap too small to reach the
+    tolerance leaves a RuntimeWarning and the reported residual above it."""
+    (ext, cells), (v, t) = _meshes()["sphere_and_slab"]
+    s = SpaceChargeSolver(ext, cells, electrode_assembly=TriangleElectrodes(v, t), solver_tol=1e-14, max_iterations=2)
+    rho = np.ones(cells) * 1e-9
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        s.solve_density(rho)
+    it, res = s.solve_stats()
+    assert it == 2 and res > 1e-14
