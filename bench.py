@@ -185,15 +185,21 @@ def lossy_tracking(w, fp, qm, nsteps=2000, radii=(3e-3, 10e-3), precision="fp32_
             out["lost_fraction"] = float(1 - active.mean())
         return out
 
-    res = {"particles": M, "steps": nsteps, "precision": precision, "pipe_length_m": length,
-           "no_mesh_generic_kernel": run(None, 500), "no_mesh_hot_kernel": run(None, 500, alive=False), "pipes": []}
-    for radius in radii:
-        pipe = TriangleElectrodes.cylinder(radius, 0.0, length, n_phi=64, n_z=256)
-        e = {"pipe_radius_m": radius, "triangles": pipe.n_triangles, "compacted": run(pipe, 500)}
-        if radius == radii[0]:
-            e["not_compacted"] = run(pipe, 0)
-        res["pipes"].append(e)
-    return res
+    pipes = [TriangleElectrodes.cylinder(radius, 0.0, length, n_phi=64, n_z=256) for radius in radii]
+
+    def measure():
+        res = {"particles": M, "steps": nsteps, "precision": precision, "pipe_length_m": length,
+               "no_mesh_generic_kernel": run(None, 500), "no_mesh_hot_kernel": run(None, 500, alive=False),
+               "pipes": []}
+        for radius, pipe in zip(radii, pipes):
+            e = {"pipe_radius_m": radius, "triangles": pipe.n_triangles, "compacted": run(pipe, 500)}
+            if radius == radii[0]:
+                e["not_compacted"] = run(pipe, 0)
+            res["pipes"].append(e)
+        return res
+
+    measure()   # untimed pass: first launches load each kernel's code and make the first allocations
+    return measure()
 
 
 def space_charge_timing(state, cells=(128, 128, 128), extent=(0.128, 0.128, 0.128), reps=3, electrodes=False):
