@@ -50,6 +50,37 @@ def test_segments_match_brute_force(kind):
     assert not np.any(got[:50] & ~want[:50])
 
 
+def test_aimed_segments_over_many_candidate_cells_match_brute_force():
+    """A 12^3 lattice of small triangles (circumradius 0.01, normal (1,1,1)) keeps each candidate cell's
+    subtree narrow, so a cell left out of a segment's candidate set loses the hit.  Segments are aimed at
+    a triangle (or just beside it) along axis, face-diagonal and body-diagonal directions with lengths up to
+    0.2: their boxes cover 1..8 cells in every shape (1x1xk, 1xkx1, kx1x1, 2x2x1, 2x2x2, ...) at the first
+    levels of the candidate pyramid."""
+    rng = np.random.default_rng(61)
+    g = (np.arange(12) + 0.5) / 12.0
+    cen = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    e1 = np.array([1.0, -1.0, 0.0]) / np.sqrt(2.0)
+    e2 = np.array([1.0, 1.0, -2.0]) / np.sqrt(6.0)
+    ang = np.array([0.0, 2.0 * np.pi / 3.0, 4.0 * np.pi / 3.0])
+    corners = 0.01 * (np.cos(ang)[:, None] * e1 + np.sin(ang)[:, None] * e2)       # (3, 3)
+    v = (cen[:, None, :] + corners[None]).reshape(-1, 3)
+    m = TriangleElectrodes(v, np.arange(len(v)).reshape(-1, 3))
+    n = 8000
+    dirs = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, 0, 1], [0, 1, 1], [1, 1, 1]], float)
+    d = dirs[rng.integers(0, len(dirs), n)] + 0.05 * rng.standard_normal((n, 3))
+    d *= rng.choice([-1.0, 1.0], (n, 1))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    t = rng.integers(0, len(cen), n)
+    off = rng.uniform(0.0, 0.02, n)[:, None] * np.where(rng.random((n, 1)) < 0.5, e1, e2)   # beside it
+    p = cen[t] + off
+    r0 = p - rng.uniform(0.001, 0.1, (n, 1)) * d
+    r1 = p + rng.uniform(0.001, 0.1, (n, 1)) * d
+    got = m.segment_intersects_surface(r0, r1)["hit_mask"]
+    want = segment_hits(m.vertices, m.triangles, r0, r1)
+    np.testing.assert_array_equal(got, want)
+    assert 0.2 < want.mean() < 0.9
+
+
 @pytest.mark.parametrize("kind", ["box", "cylinder", "soup"])
 def test_short_segments_near_surfaces_match_brute_force(kind):
     """Segments of 1e-7 .. 1e-2 of the mesh size around the surface (the occupancy grid decides most of
