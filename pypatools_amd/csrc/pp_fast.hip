@@ -240,15 +240,20 @@ DEV d3 kick_mixed(int64_t i, d3 r, d3 v, double dt, int64_t gs, const FE& fe, co
     else return vay_kick(v, Ed, Bd, dt, p.qm, p.c2, p.clight);
 }
 
-template <int ALG, bool LOSS>
-__global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    if (p.alive && !p.alive[i]) return;
+// E and B of uniform terms without an RF factor (uniform_term): the eval_fields32 values, taken once.
+struct ConstFE32 {
+    f3 E0, B0;
+    DEV void operator()(int64_t, double, double, double, int64_t, int, f3& E, f3& B) const {
+        E = E0;
+        B = B0;
+    }
+};
+
+template <int ALG, bool LOSS, class FE>
+DEV void push_mixed_particle(const PushParams& p, int64_t i, const FE& fe) {
     d3 r = mk3(p.x[i], p.y[i], p.z[i]);
     d3 v = mk3(p.vx[i], p.vy[i], p.vz[i]);
     const double dt = p.dt_pp ? p.dt_pp[i] : p.dt;
-    const GenericFE32 fe{p};
     const AlgConst ac = alg_const(p);
     if (p.kick_only) {
         if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
@@ -285,6 +290,20 @@ __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     p.vx[i] = v.x;
     p.vy[i] = v.y;
     p.vz[i] = v.z;
+}
+
+// UNIFORM: both terms are uniform; their values are taken once per particle.
+template <int ALG, bool LOSS, bool UNIFORM = false>
+__global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    if (p.alive && !p.alive[i]) return;
+    if constexpr (UNIFORM) {
+        const f3 zero = mkf(0.f, 0.f, 0.f);
+        push_mixed_particle<ALG, LOSS>(p, i, ConstFE32{finish_term32(p.E, zero, 1.f), finish_term32(p.B, zero, 1.f)});
+    } else {
+        push_mixed_particle<ALG, LOSS>(p, i, GenericFE32{p});
+    }
 }
 
 __global__ void __launch_bounds__(256) k_field_eval32(const FieldDev f, const double* __restrict__ pts, int64_t m,
@@ -772,17 +791,17 @@ hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int blo
     return hipGetLastError();
 }
 
-#define PP_FAST_CASES(KERNEL, L)                                                                          \
+#define PP_FAST_CASES(KERNEL, L, U)                                                                          \
     switch (alg) {                                                                                        \
-        case PP_ALG_LEAPFROG: hipLaunchKernelGGL((KERNEL<PP_ALG_LEAPFROG, L>), g, b, 0, s, p); break;     \
-        case PP_ALG_BORIS: hipLaunchKernelGGL((KERNEL<PP_ALG_BORIS, L>), g, b, 0, s, p); break;           \
-        case PP_ALG_RK4: hipLaunchKernelGGL((KERNEL<PP_ALG_RK4, L>), g, b, 0, s, p); break;               \
-        case PP_ALG_YOSHIDA: hipLaunchKernelGGL((KERNEL<PP_ALG_YOSHIDA, L>), g, b, 0, s, p); break;       \
-        case PP_ALG_VAY_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_VAY_REL, L>), g, b, 0, s, p); break;       \
-        case PP_ALG_RK4_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_RK4_REL, L>), g, b, 0, s, p); break;       \
-        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_YOSHIDA_REL, L>), g, b, 0, s, p); break; \
-        case PP_ALG_BORIS_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_BORIS_REL, L>), g, b, 0, s, p); break;   \
-        case PP_ALG_VAY: hipLaunchKernelGGL((KERNEL<PP_ALG_VAY, L>), g, b, 0, s, p); break;               \
+        case PP_ALG_LEAPFROG: hipLaunchKernelGGL((KERNEL<PP_ALG_LEAPFROG, L, U>), g, b, 0, s, p); break;     \
+        case PP_ALG_BORIS: hipLaunchKernelGGL((KERNEL<PP_ALG_BORIS, L, U>), g, b, 0, s, p); break;           \
+        case PP_ALG_RK4: hipLaunchKernelGGL((KERNEL<PP_ALG_RK4, L, U>), g, b, 0, s, p); break;               \
+        case PP_ALG_YOSHIDA: hipLaunchKernelGGL((KERNEL<PP_ALG_YOSHIDA, L, U>), g, b, 0, s, p); break;       \
+        case PP_ALG_VAY_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_VAY_REL, L, U>), g, b, 0, s, p); break;       \
+        case PP_ALG_RK4_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_RK4_REL, L, U>), g, b, 0, s, p); break;       \
+        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_YOSHIDA_REL, L, U>), g, b, 0, s, p); break; \
+        case PP_ALG_BORIS_REL: hipLaunchKernelGGL((KERNEL<PP_ALG_BORIS_REL, L, U>), g, b, 0, s, p); break;   \
+        case PP_ALG_VAY: hipLaunchKernelGGL((KERNEL<PP_ALG_VAY, L, U>), g, b, 0, s, p); break;               \
         default: return hipErrorInvalidValue;                                                             \
     }
 
@@ -790,17 +809,22 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
     const bool loss = p.mesh != nullptr;
+    const bool uniform = !loss && !p.fused && uniform_term(p.E) && uniform_term(p.B);
     if (prec == PP_PREC_FP64_FAST) {
         if (loss) {
-            PP_FAST_CASES(k_push64, true)
+            PP_FAST_CASES(k_push64, true, false)
+        } else if (uniform) {
+            PP_FAST_CASES(k_push64, false, true)
         } else {
-            PP_FAST_CASES(k_push64, false)
+            PP_FAST_CASES(k_push64, false, false)
         }
     } else {
         if (loss) {
-            PP_FAST_CASES(k_push_mixed, true)
+            PP_FAST_CASES(k_push_mixed, true, false)
+        } else if (uniform) {
+            PP_FAST_CASES(k_push_mixed, false, true)
         } else {
-            PP_FAST_CASES(k_push_mixed, false)
+            PP_FAST_CASES(k_push_mixed, false, false)
         }
     }
     return hipGetLastError();

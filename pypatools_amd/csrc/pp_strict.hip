@@ -131,17 +131,17 @@ __global__ void __launch_bounds__(256) k_build_coef(const MapDesc d, double* __r
 
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-#define PP_STRICT_CASES(L)                                                                                \
+#define PP_STRICT_CASES(L, U)                                                                                \
     switch (alg) {                                                                                        \
-        case PP_ALG_LEAPFROG: hipLaunchKernelGGL((k_push64<PP_ALG_LEAPFROG, L>), g, b, 0, s, p); break;   \
-        case PP_ALG_BORIS: hipLaunchKernelGGL((k_push64<PP_ALG_BORIS, L>), g, b, 0, s, p); break;         \
-        case PP_ALG_RK4: hipLaunchKernelGGL((k_push64<PP_ALG_RK4, L>), g, b, 0, s, p); break;             \
-        case PP_ALG_YOSHIDA: hipLaunchKernelGGL((k_push64<PP_ALG_YOSHIDA, L>), g, b, 0, s, p); break;     \
-        case PP_ALG_VAY_REL: hipLaunchKernelGGL((k_push64<PP_ALG_VAY_REL, L>), g, b, 0, s, p); break;     \
-        case PP_ALG_RK4_REL: hipLaunchKernelGGL((k_push64<PP_ALG_RK4_REL, L>), g, b, 0, s, p); break;     \
-        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL((k_push64<PP_ALG_YOSHIDA_REL, L>), g, b, 0, s, p); break; \
-        case PP_ALG_BORIS_REL: hipLaunchKernelGGL((k_push64<PP_ALG_BORIS_REL, L>), g, b, 0, s, p); break; \
-        case PP_ALG_VAY: hipLaunchKernelGGL((k_push64<PP_ALG_VAY, L>), g, b, 0, s, p); break;             \
+        case PP_ALG_LEAPFROG: hipLaunchKernelGGL((k_push64<PP_ALG_LEAPFROG, L, U>), g, b, 0, s, p); break;   \
+        case PP_ALG_BORIS: hipLaunchKernelGGL((k_push64<PP_ALG_BORIS, L, U>), g, b, 0, s, p); break;         \
+        case PP_ALG_RK4: hipLaunchKernelGGL((k_push64<PP_ALG_RK4, L, U>), g, b, 0, s, p); break;             \
+        case PP_ALG_YOSHIDA: hipLaunchKernelGGL((k_push64<PP_ALG_YOSHIDA, L, U>), g, b, 0, s, p); break;     \
+        case PP_ALG_VAY_REL: hipLaunchKernelGGL((k_push64<PP_ALG_VAY_REL, L, U>), g, b, 0, s, p); break;     \
+        case PP_ALG_RK4_REL: hipLaunchKernelGGL((k_push64<PP_ALG_RK4_REL, L, U>), g, b, 0, s, p); break;     \
+        case PP_ALG_YOSHIDA_REL: hipLaunchKernelGGL((k_push64<PP_ALG_YOSHIDA_REL, L, U>), g, b, 0, s, p); break; \
+        case PP_ALG_BORIS_REL: hipLaunchKernelGGL((k_push64<PP_ALG_BORIS_REL, L, U>), g, b, 0, s, p); break; \
+        case PP_ALG_VAY: hipLaunchKernelGGL((k_push64<PP_ALG_VAY, L, U>), g, b, 0, s, p); break;             \
         default: return hipErrorInvalidValue;                                                             \
     }
 
@@ -149,9 +149,11 @@ hipError_t launch_push(int alg, const PushParams& p, int block, hipStream_t s) {
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
     if (p.mesh) {
-        PP_STRICT_CASES(true)
+        PP_STRICT_CASES(true, false)
+    } else if (!p.fused && uniform_term(p.E) && uniform_term(p.B)) {
+        PP_STRICT_CASES(false, true)
     } else {
-        PP_STRICT_CASES(false)
+        PP_STRICT_CASES(false, false)
     }
     return hipGetLastError();
 }
