@@ -66,24 +66,29 @@ DEV f3 map_components32(const FieldDev& f, double px, double py, double pz) {
     return mkf(coef_eval32(q, tx, ty, tz), coef_eval32(q + 8, tx, ty, tz), coef_eval32(q + 16, tx, ty, tz));
 }
 
+// E and B of a fused six-component map at one position (one cell lookup for both).
+DEV void fused_components32(const PushParams& p, double px, double py, double pz, f3& e0, f3& b0) {
+    const FastCell c = fast_locate(p.E.map, p.E.map.coef32, px, py, pz);
+    const float fl = (float)p.E.map.fill;
+    e0 = mkf(fl, fl, fl);
+    b0 = e0;
+    if (c.inside) {
+        const float* q = static_cast<const float*>(c.blk);
+        const float tx = (float)c.tx, ty = (float)c.ty, tz = (float)c.tz;
+        const float* qe = q + p.E.comp0 * 8;
+        const float* qb = q + p.B.comp0 * 8;
+        e0 = mkf(coef_eval32(qe, tx, ty, tz), coef_eval32(qe + 8, tx, ty, tz), coef_eval32(qe + 16, tx, ty, tz));
+        b0 = mkf(coef_eval32(qb, tx, ty, tz), coef_eval32(qb + 8, tx, ty, tz), coef_eval32(qb + 16, tx, ty, tz));
+    }
+}
+
 DEV void eval_fields32(const PushParams& p, int64_t i, double px, double py, double pz, int64_t gs, int stage, f3& E,
                        f3& B) {
     const float rfE = p.E.rf != PP_RF_NONE ? (float)rf_factor(p, gs, stage, p.E.rf) : 1.f;
     const float rfB = p.B.rf != PP_RF_NONE ? (float)rf_factor(p, gs, stage, p.B.rf) : 1.f;
     f3 e0 = mkf(0.f, 0.f, 0.f), b0 = mkf(0.f, 0.f, 0.f);
     if (p.fused) {
-        const FastCell c = fast_locate(p.E.map, p.E.map.coef32, px, py, pz);
-        const float fl = (float)p.E.map.fill;
-        e0 = mkf(fl, fl, fl);
-        b0 = e0;
-        if (c.inside) {
-            const float* q = static_cast<const float*>(c.blk);
-            const float tx = (float)c.tx, ty = (float)c.ty, tz = (float)c.tz;
-            const float* qe = q + p.E.comp0 * 8;
-            const float* qb = q + p.B.comp0 * 8;
-            e0 = mkf(coef_eval32(qe, tx, ty, tz), coef_eval32(qe + 8, tx, ty, tz), coef_eval32(qe + 16, tx, ty, tz));
-            b0 = mkf(coef_eval32(qb, tx, ty, tz), coef_eval32(qb + 8, tx, ty, tz), coef_eval32(qb + 16, tx, ty, tz));
-        }
+        fused_components32(p, px, py, pz, e0, b0);
     } else {
         if (p.E.kind == PP_FIELD_MAP) e0 = map_components32(p.E, px, py, pz);
         else if (p.E.kind == PP_FIELD_ARRAY) e0 = tof(load3(p.E.array, i));
@@ -292,15 +297,29 @@ DEV void push_mixed_particle(const PushParams& p, int64_t i, const FE& fe) {
     p.vz[i] = v.z;
 }
 
-// UNIFORM: both terms are uniform; their values are taken once per particle.
-template <int ALG, bool LOSS, bool UNIFORM = false>
+// E and B of one fused map with no RF factor on either term (eval_fields32 without the other kinds).
+struct FusedFE32 {
+    const PushParams& p;
+    DEV void operator()(int64_t, double x, double y, double z, int64_t, int, f3& E, f3& B) const {
+        f3 e0, b0;
+        fused_components32(p, x, y, z, e0, b0);
+        E = finish_term32(p.E, e0, 1.f);
+        B = finish_term32(p.B, b0, 1.f);
+    }
+};
+
+// FIELDS (fields_case): uniform terms are taken once per particle; a fused static map
+// keeps only its own code in the step loop.
+template <int ALG, bool LOSS, int FIELDS = kFieldsGeneral>
 __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     if (p.alive && !p.alive[i]) return;
-    if constexpr (UNIFORM) {
+    if constexpr (FIELDS == kFieldsUniform) {
         const f3 zero = mkf(0.f, 0.f, 0.f);
         push_mixed_particle<ALG, LOSS>(p, i, ConstFE32{finish_term32(p.E, zero, 1.f), finish_term32(p.B, zero, 1.f)});
+    } else if constexpr (FIELDS == kFieldsFusedStatic) {
+        push_mixed_particle<ALG, LOSS>(p, i, FusedFE32{p});
     } else {
         push_mixed_particle<ALG, LOSS>(p, i, GenericFE32{p});
     }
@@ -809,22 +828,29 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
     const bool loss = p.mesh != nullptr;
-    const bool uniform = !loss && !p.fused && uniform_term(p.E) && uniform_term(p.B);
+    const int fields = fields_case(p);
+    // the loss test runs with the general or the fused static evaluator
     if (prec == PP_PREC_FP64_FAST) {
         if (loss) {
-            PP_FAST_CASES(k_push64, true, false)
-        } else if (uniform) {
-            PP_FAST_CASES(k_push64, false, true)
+            if (fields == kFieldsFusedStatic) PP_FAST_CASES(k_push64, true, kFieldsFusedStatic)
+            else PP_FAST_CASES(k_push64, true, kFieldsGeneral)
+        } else if (fields == kFieldsUniform) {
+            PP_FAST_CASES(k_push64, false, kFieldsUniform)
+        } else if (fields == kFieldsFusedStatic) {
+            PP_FAST_CASES(k_push64, false, kFieldsFusedStatic)
         } else {
-            PP_FAST_CASES(k_push64, false, false)
+            PP_FAST_CASES(k_push64, false, kFieldsGeneral)
         }
     } else {
         if (loss) {
-            PP_FAST_CASES(k_push_mixed, true, false)
-        } else if (uniform) {
-            PP_FAST_CASES(k_push_mixed, false, true)
+            if (fields == kFieldsFusedStatic) PP_FAST_CASES(k_push_mixed, true, kFieldsFusedStatic)
+            else PP_FAST_CASES(k_push_mixed, true, kFieldsGeneral)
+        } else if (fields == kFieldsUniform) {
+            PP_FAST_CASES(k_push_mixed, false, kFieldsUniform)
+        } else if (fields == kFieldsFusedStatic) {
+            PP_FAST_CASES(k_push_mixed, false, kFieldsFusedStatic)
         } else {
-            PP_FAST_CASES(k_push_mixed, false, false)
+            PP_FAST_CASES(k_push_mixed, false, kFieldsGeneral)
         }
     }
     return hipGetLastError();

@@ -148,12 +148,16 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + b
 hipError_t launch_push(int alg, const PushParams& p, int block, hipStream_t s) {
     if (block <= 0) block = 256;
     const dim3 g(grid_for(p.n, block)), b(block);
+    const int fields = fields_case(p);
     if (p.mesh) {
-        PP_STRICT_CASES(true, false)
-    } else if (!p.fused && uniform_term(p.E) && uniform_term(p.B)) {
-        PP_STRICT_CASES(false, true)
+        if (fields == kFieldsFusedStatic) PP_STRICT_CASES(true, kFieldsFusedStatic)
+        else PP_STRICT_CASES(true, kFieldsGeneral)
+    } else if (fields == kFieldsUniform) {
+        PP_STRICT_CASES(false, kFieldsUniform)
+    } else if (fields == kFieldsFusedStatic) {
+        PP_STRICT_CASES(false, kFieldsFusedStatic)
     } else {
-        PP_STRICT_CASES(false, false)
+        PP_STRICT_CASES(false, kFieldsGeneral)
     }
     return hipGetLastError();
 }
