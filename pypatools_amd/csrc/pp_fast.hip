@@ -838,6 +838,8 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
             PP_FAST_CASES(k_push64, false, kFieldsUniform)
         } else if (fields == kFieldsFusedStatic) {
             PP_FAST_CASES(k_push64, false, kFieldsFusedStatic)
+        } else if (fields == kFieldsFusedRF) {
+            PP_FAST_CASES(k_push64, false, kFieldsFusedRF)
         } else {
             PP_FAST_CASES(k_push64, false, kFieldsGeneral)
         }

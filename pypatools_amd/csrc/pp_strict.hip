@@ -156,6 +156,8 @@ hipError_t launch_push(int alg, const PushParams& p, int block, hipStream_t s) {
         PP_STRICT_CASES(false, kFieldsUniform)
     } else if (fields == kFieldsFusedStatic) {
         PP_STRICT_CASES(false, kFieldsFusedStatic)
+    } else if (fields == kFieldsFusedRF) {
+        PP_STRICT_CASES(false, kFieldsFusedRF)
     } else {
         PP_STRICT_CASES(false, kFieldsGeneral)
     }
