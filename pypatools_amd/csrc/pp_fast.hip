@@ -308,8 +308,21 @@ struct FusedFE32 {
     }
 };
 
-// FIELDS (fields_case): uniform terms are taken once per particle; a fused static map
-// keeps only its own code in the step loop.
+// The same with the RF factors of the terms (eval_fields32 for a fused map).
+struct FusedRFFE32 {
+    const PushParams& p;
+    DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B) const {
+        const float rfE = p.E.rf != PP_RF_NONE ? (float)rf_factor(p, gs, stage, p.E.rf) : 1.f;
+        const float rfB = p.B.rf != PP_RF_NONE ? (float)rf_factor(p, gs, stage, p.B.rf) : 1.f;
+        f3 e0, b0;
+        fused_components32(p, x, y, z, e0, b0);
+        E = finish_term32(p.E, e0, rfE);
+        B = finish_term32(p.B, b0, rfB);
+    }
+};
+
+// FIELDS (fields_case): uniform terms are taken once per particle; a fused map keeps
+// only its own code in the step loop.
 template <int ALG, bool LOSS, int FIELDS = kFieldsGeneral>
 __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,6 +333,8 @@ __global__ void __launch_bounds__(256) k_push_mixed(const PushParams p) {
         push_mixed_particle<ALG, LOSS>(p, i, ConstFE32{finish_term32(p.E, zero, 1.f), finish_term32(p.B, zero, 1.f)});
     } else if constexpr (FIELDS == kFieldsFusedStatic) {
         push_mixed_particle<ALG, LOSS>(p, i, FusedFE32{p});
+    } else if constexpr (FIELDS == kFieldsFusedRF) {
+        push_mixed_particle<ALG, LOSS>(p, i, FusedRFFE32{p});
     } else {
         push_mixed_particle<ALG, LOSS>(p, i, GenericFE32{p});
     }
@@ -829,10 +844,11 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
     const dim3 g(grid_for(p.n, block)), b(block);
     const bool loss = p.mesh != nullptr;
     const int fields = fields_case(p);
-    // the loss test runs with the general or the fused static evaluator
+    // the loss test runs with the general or a fused-map evaluator
     if (prec == PP_PREC_FP64_FAST) {
         if (loss) {
             if (fields == kFieldsFusedStatic) PP_FAST_CASES(k_push64, true, kFieldsFusedStatic)
+            else if (fields == kFieldsFusedRF) PP_FAST_CASES(k_push64, true, kFieldsFusedRF)
             else PP_FAST_CASES(k_push64, true, kFieldsGeneral)
         } else if (fields == kFieldsUniform) {
             PP_FAST_CASES(k_push64, false, kFieldsUniform)
@@ -846,11 +862,14 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
     } else {
         if (loss) {
             if (fields == kFieldsFusedStatic) PP_FAST_CASES(k_push_mixed, true, kFieldsFusedStatic)
+            else if (fields == kFieldsFusedRF) PP_FAST_CASES(k_push_mixed, true, kFieldsFusedRF)
             else PP_FAST_CASES(k_push_mixed, true, kFieldsGeneral)
         } else if (fields == kFieldsUniform) {
             PP_FAST_CASES(k_push_mixed, false, kFieldsUniform)
         } else if (fields == kFieldsFusedStatic) {
             PP_FAST_CASES(k_push_mixed, false, kFieldsFusedStatic)
+        } else if (fields == kFieldsFusedRF) {
+            PP_FAST_CASES(k_push_mixed, false, kFieldsFusedRF)
         } else {
             PP_FAST_CASES(k_push_mixed, false, kFieldsGeneral)
         }

@@ -151,6 +151,7 @@ hipError_t launch_push(int alg, const PushParams& p, int block, hipStream_t s) {
     const int fields = fields_case(p);
     if (p.mesh) {
         if (fields == kFieldsFusedStatic) PP_STRICT_CASES(true, kFieldsFusedStatic)
+        else if (fields == kFieldsFusedRF) PP_STRICT_CASES(true, kFieldsFusedRF)
         else PP_STRICT_CASES(true, kFieldsGeneral)
     } else if (fields == kFieldsUniform) {
         PP_STRICT_CASES(false, kFieldsUniform)
