@@ -27,5 +27,51 @@ The oracle is a NumPy FP64 restatement of PyPATools' pushers and field gather:
 * ``pusher``     -- Pusher.push / push_batch / track / track_batch orchestration
   (src/PyPATools/pusher.py:430-1037).
 
-Pinning status of each function is listed in ``oracle/PINNING.md``.
+How each part is pinned (checks that do not restate the oracle's own formulas):
+
+=====================================  =====================================================
+oracle function                        pinned by (tests/...)
+=====================================  =====================================================
+species.q_over_m, constants            golden q/m written by the imported reference
+                                       (test_oracle_golden.py); CODATA values
+                                       (test_oracle_closed_forms.py)
+kernels.boris_kick / boris_step        discrete-rotation and constant-E closed forms;
+                                       golden track_batch / track / push vectors
+kernels.vay_rel_kick / vay_rel_step    golden vay_push_single vectors (bitwise, as-shipped);
+                                       speed-growth and clamp invariants
+kernels.rk4_step (rk4, rk4_rel)        4th-order convergence; 1-D RF closed form;
+                                       relativistic gyration; golden tracks
+kernels.leapfrog_step, yoshida_step    constant-E closed form; Yoshida position-sum quirk;
+                                       golden tracks
+kernels.relativistic_gamma,            golden vectors with beta up to 0.999999 (clamp);
+clamp_speed                            vay_rel clamp fixed point
+kernels.boris_rel_kick_ext             half-kick / exact-rotation / half-kick closed form
+                                       on random fields; pure-E kick u_N = u_0 + (q/m) E N dt;
+                                       E parallel to B; relativistic gyro angle
+kernels.vay_kick_ext                   exact E x B drift; relativistic gyro angle
+interp.interp_nd                       exact on multilinear fields; scipy
+                                       RegularGridInterpolator and map_coordinates;
+                                       golden Field.__call__ outputs; face / outside points
+interp.interp_coordinate_mapper        golden 'fast' backend outputs (bitwise)
+field.OField / OScaled / OComposite    golden Field.__call__ outputs for every field kind,
+                                       backend and the field algebra
+field.OPeriodicZ (extension)           wrap cell = (1 - f) slice[nz-1] + f slice[0] by
+                                       scipy on the slices, z0 != 0, below z0, k periods
+                                       away; shift invariance
+field.ORF (extension)                  1-D RF closed form of rk4_rel
+pusher.OraclePusher                    golden track_batch / track / push / push_batch
+                                       vectors of every reference algorithm, including
+                                       particle loss and the middle-loss quirk
+moments.*                              Twiss identities, emittance invariance under thin
+                                       lens and drift, energy closed form
+electrodes.segment_hits                plate closed forms; closed-box parity
+space_charge.*                         discrete eigenfunction; charge conservation; exact
+                                       gradients of linear potentials; half-space and
+                                       polyhedron ray geometry; Shortley-Weller rows at a
+                                       plane; second-order manufactured-solution convergence
+=====================================  =====================================================
+
+Parity unpinned: py_electrodes' own hit rule and ray query (the package is not
+available; the triangle-mesh semantics above stand in for it), and the H5Part
+parse (h5py is not available).
 """

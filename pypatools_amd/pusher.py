@@ -1,13 +1,15 @@
 """Pusher: the reference's particle-pusher API on the HIP engine.
 
 Same surface as src/PyPATools/pusher.py:430-1037: algorithm chosen by name
-(``ALGORITHMS``; unknown names raise the same ValueError), ``push`` /
+(``ALGORITHMS``, the reference's seven names; unknown names raise the same
+ValueError with the same text), ``push`` /
 ``push_batch`` / ``track`` / ``track_batch`` / ``change_algorithm``, host
 float64 (M, 3) arrays in and out.  All arithmetic runs in the HIP kernels of
 libpypat_hip.so; ``use_numba`` is accepted and ignored.
 
 Additions: algorithms 'boris_rel' (relativistic Boris) and 'vay' (Vay 2008),
-both staggered like 'boris'; ``precision`` ('fp64' = bitwise the reference's
+both staggered like 'boris', listed in ``EXTENSION_ALGORITHMS`` and accepted
+by name beside ``ALGORITHMS``; ``precision`` ('fp64' = bitwise the reference's
 FP64 NumPy arithmetic, 'fp64_fast', 'fp32_mixed'); ``t0`` for time-dependent
 fields (RFField); per-particle ``dt`` arrays for the RK4 family
 (pusher.py:815-816).
@@ -23,15 +25,15 @@ from .engine import EXTENSION_ALGORITHMS, REFERENCE_ALGORITHMS, Tracker
 class Pusher:
     """Particle pusher for tracking charged particles through EM fields."""
 
-    ALGORITHMS = REFERENCE_ALGORITHMS + EXTENSION_ALGORITHMS
+    ALGORITHMS = list(REFERENCE_ALGORITHMS)           # pusher.py:465-466
+    EXTENSION_ALGORITHMS = list(EXTENSION_ALGORITHMS)
 
     def __init__(self, ion, algorithm="boris", use_numba=True, electrode_assembly=None, precision="fp64",
                  block_size=0, sort_every=None):
         self.ion = ion
         self.q_over_m = ion.q_over_m
         algorithm = algorithm.lower()
-        if algorithm not in self.ALGORITHMS:
-            raise ValueError(f"Unknown algorithm '{algorithm}'. Must be one of {self.ALGORITHMS}")
+        self._check_name(algorithm)
         # pusher.py:480-489: a leapfrog request for an ion that carries an energy above 100 keV
         if algorithm == "leapfrog" and hasattr(ion, "energy_mev"):
             if ion.energy_mev > 0.1:
@@ -47,6 +49,11 @@ class Pusher:
         self.precision = precision
         self.block_size = block_size
         self.sort_every = sort_every
+
+    def _check_name(self, algorithm):
+        """pusher.py:475-478: the message lists ALGORITHMS (the extensions are accepted, not listed)."""
+        if algorithm not in self.ALGORITHMS and algorithm not in self.EXTENSION_ALGORITHMS:
+            raise ValueError(f"Unknown algorithm '{algorithm}'. Must be one of {self.ALGORITHMS}")
 
     def _tracker(self):
         kw = {} if self.sort_every is None else {"sort_every": self.sort_every}
@@ -91,11 +98,12 @@ class Pusher:
     def change_algorithm(self, algorithm):
         """pusher.py:1018-1033 (like the reference, the electrode assembly is not carried over)."""
         algorithm = algorithm.lower()
-        if algorithm not in self.ALGORITHMS:
-            raise ValueError(f"Unknown algorithm '{algorithm}'. Must be one of {self.ALGORITHMS}")
+        self._check_name(algorithm)
         self.__init__(self.ion, algorithm=algorithm, use_numba=self.use_numba, precision=self.precision,
                       block_size=self.block_size, sort_every=self.sort_every)
 
     def __repr__(self):
+        """pusher.py:1035-1037; a precision other than the default 'fp64' is appended."""
+        extra = "" if self.precision == "fp64" else f", precision='{self.precision}'"
         return (f"Pusher(ion={self.ion.name}, algorithm='{self.algorithm}', "
-                f"relativistic={self.relativistic}, numba={self.use_numba}, precision='{self.precision}')")
+                f"relativistic={self.relativistic}, numba={self.use_numba}{extra})")

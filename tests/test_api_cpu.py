@@ -43,13 +43,21 @@ def test_pusher_names_errors_and_repr():
     assert p.algorithm == "boris" and not p.relativistic
     assert pa.Pusher(_Ion(), "vay_rel").relativistic
     assert pa.Pusher(_Ion(), "vay").relativistic
-    with pytest.raises(ValueError, match=r"Unknown algorithm 'euler'\. Must be one of \["):
+    ref_names = ["leapfrog", "boris", "rk4", "yoshida", "vay_rel", "rk4_rel", "yoshida_rel"]
+    assert pa.Pusher.ALGORITHMS == ref_names                       # pusher.py:465-466, same order
+    assert pa.Pusher.EXTENSION_ALGORITHMS == ["boris_rel", "vay"]
+    with pytest.raises(ValueError) as err:
         pa.Pusher(_Ion(), "euler")
+    assert str(err.value) == ("Unknown algorithm 'euler'. Must be one of ['leapfrog', 'boris', 'rk4', 'yoshida', "
+                              "'vay_rel', 'rk4_rel', 'yoshida_rel']")
+    with pytest.raises(ValueError) as err:
+        p.change_algorithm("Euler")
+    assert str(err.value).startswith("Unknown algorithm 'euler'. Must be one of ['leapfrog'")
     with pytest.raises(ValueError):
         pa.Pusher(_Ion(), "boris", precision="fp16")
-    assert repr(p).startswith("Pusher(ion=H2_1+, algorithm='boris', relativistic=False")
-    for a in pa.Pusher.ALGORITHMS[:7]:
-        assert a in ["leapfrog", "boris", "rk4", "yoshida", "vay_rel", "rk4_rel", "yoshida_rel"]
+    assert repr(p) == "Pusher(ion=H2_1+, algorithm='boris', relativistic=False, numba=True)"
+    assert repr(pa.Pusher(_Ion(), "vay", precision="fp32_mixed")) == (
+        "Pusher(ion=H2_1+, algorithm='vay', relativistic=True, numba=True, precision='fp32_mixed')")
 
 
 def test_change_algorithm_drops_electrode_assembly():

@@ -28,7 +28,7 @@ def _device():
         pytest.skip("no HIP device visible")
 
 
-ALGS = pa.Pusher.ALGORITHMS
+ALGS = pa.Pusher.ALGORITHMS + pa.Pusher.EXTENSION_ALGORITHMS
 QM_P = OS.q_over_m("proton")
 QM_E = OS.q_over_m("electron")
 

@@ -172,8 +172,27 @@ def test_relativistic_gyro_angle_and_norm(alg):
     assert np.allclose(spd, beta * CLIGHT, rtol=1e-13)
 
 
-def test_vay_exact_ExB_drift_and_boris_rel_error():
-    """Vay (2008) reproduces the E x B drift exactly for any gamma; relativistic Boris does not."""
+def _rotate(u, axis, angle):
+    """Rodrigues rotation of the rows of u by `angle` (right-handed) about the unit vectors `axis`."""
+    c, s = np.cos(angle)[:, None], np.sin(angle)[:, None]
+    return u * c + np.cross(axis, u) * s + axis * np.sum(axis * u, axis=1)[:, None] * (1.0 - c)
+
+
+def _boris_rel_closed_form(u, E, B, dt, qm):
+    """Relativistic Boris as its geometric definition: half E kick, exact rotation of u about -B by
+    2 atan(|q/m| |B| dt / (2 gamma_-)), half E kick (Birdsall & Langdon 15-4)."""
+    dE = 0.5 * qm * dt * E
+    um = u + dE
+    gm = np.sqrt(1.0 + np.sum(um * um, axis=1) / CLIGHT ** 2)
+    bn = np.linalg.norm(B, axis=1)
+    ang = 2.0 * np.arctan(np.abs(qm) * bn * dt / (2.0 * gm))
+    axis = -np.sign(qm) * B / np.where(bn > 0, bn, 1.0)[:, None]
+    return _rotate(um, axis, ang) + dE
+
+
+def test_vay_exact_ExB_drift_and_boris_rel_closed_form():
+    """Vay (2008) reproduces the E x B drift exactly for any gamma.  Relativistic Boris does not;
+    its step equals its geometric definition (half kick, exact rotation, half kick) to rounding."""
     qm = OS.q_over_m("proton")
     B = np.array([0.0, 0.0, 1.0])
     E = np.array([0.0, 0.9 * CLIGHT, 0.0])          # |E| = 0.9 c |B|: E x B / B^2 = 0.9 c along +x
@@ -185,8 +204,52 @@ def test_vay_exact_ExB_drift_and_boris_rel_error():
         rn, vn = pu.push_batch(np.zeros((1, 3)), v0, const(E), const(B), 1e-9)
         outs[alg] = vn
     assert np.allclose(outs["vay"], v0, rtol=1e-12, atol=1e-6)
-    # relativistic Boris mixes the E and B parts at gamma_d ~ 2.3: error ~1e-5 per step
-    assert np.linalg.norm(outs["boris_rel"] - v0) > 1e-6 * np.linalg.norm(v0)
+    u1 = _boris_rel_closed_form(K.u_from_v(v0), E[None, :], B[None, :], 1e-9, qm)
+    assert np.linalg.norm(outs["boris_rel"] - K.v_from_u(u1)) < 1e-13 * np.linalg.norm(v0)
+    assert np.linalg.norm(outs["boris_rel"] - v0) > 1e-6 * np.linalg.norm(v0)   # not the drift
+
+
+def test_boris_rel_kick_matches_geometric_definition():
+    """Random u, E, B over four decades of field strength: the oracle's relativistic Boris step equals
+    the half-kick / exact-rotation / half-kick closed form, for both charge signs."""
+    rng = np.random.default_rng(12)
+    M = 2000
+    u = rng.standard_normal((M, 3)) * 0.5 * CLIGHT
+    E = rng.standard_normal((M, 3)) * 10.0 ** rng.uniform(4, 8, (M, 1))
+    B = rng.standard_normal((M, 3)) * 10.0 ** rng.uniform(-2, 1, (M, 1))
+    for qm, dt in ((QM_P, 1e-10), (OS.q_over_m("electron"), 1e-13)):
+        got = K.boris_rel_kick_ext(u, E, B, dt, qm)
+        want = _boris_rel_closed_form(u, E, B, dt, qm)
+        scale = np.linalg.norm(u, axis=1) + np.linalg.norm(0.5 * qm * dt * E, axis=1)
+        assert np.max(np.linalg.norm(got - want, axis=1) / scale) < 1e-13
+
+
+def test_boris_rel_pure_E_and_E_parallel_B_kicks_are_exact():
+    """With B = 0, relativistic Boris gives u_N = u_0 + (q/m) E N dt exactly (the two half kicks add to
+    one full kick).  With E parallel to B the kick along B is unchanged by the rotation: u_par grows by
+    (q/m) E dt per step and |u_perp| is conserved.  A wrong E factor or a dropped half kick fails both."""
+    qm, dt, N = OS.q_over_m("electron"), 1e-12, 1000
+    rng = np.random.default_rng(13)
+    u0 = rng.standard_normal((8, 3)) * 1e8
+    E = np.array([[2e5, -1e5, 3e5]])
+    u = u0.copy()
+    for _ in range(N):
+        u = K.boris_rel_kick_ext(u, E, np.zeros((1, 3)), dt, qm)
+    want = u0 + qm * E * (N * dt)
+    assert np.max(np.linalg.norm(u - want, axis=1) / np.linalg.norm(want, axis=1)) < 1e-12
+    # the same through track_batch: v_hist holds v(u) of the staggered scheme at every record
+    rh, vh, _ = OraclePusher(qm, "boris_rel").track_batch(np.zeros((8, 3)), K.v_from_u(u0), const(E[0]),
+                                                           const((0.0, 0.0, 0.0)), N, dt, N)
+    u_back = K.u_from_v(vh[-1])
+    assert np.max(np.linalg.norm(u_back - want, axis=1) / np.linalg.norm(want, axis=1)) < 1e-10
+    # E parallel to B along z
+    Ez, Bz = 4e5, 0.7
+    u = u0.copy()
+    for _ in range(N):
+        u = K.boris_rel_kick_ext(u, np.array([[0.0, 0.0, Ez]]), np.array([[0.0, 0.0, Bz]]), dt, qm)
+    scale = np.linalg.norm(u0, axis=1) + abs(qm * Ez * N * dt)
+    assert np.max(np.abs(u[:, 2] - (u0[:, 2] + qm * Ez * N * dt)) / scale) < 1e-12
+    assert np.max(np.abs(np.hypot(u[:, 0], u[:, 1]) - np.hypot(u0[:, 0], u0[:, 1])) / scale) < 1e-12
 
 
 def test_vay_rel_speed_growth_quirk():
@@ -266,3 +329,36 @@ def test_periodic_z_wrap_matches_shifted_evaluation():
     inner = pts[pts[:, 2] <= grid["z"][-1]]
     plain = OField.from_arrays(grid, vals)
     assert np.array_equal(f(inner), plain(inner))
+
+
+@pytest.mark.parametrize("z0", [0.0, 0.0137, -0.25])
+def test_periodic_wrap_cell_interpolates_last_and_first_slices(z0):
+    """The wrap cell (z_last, z0 + L) joins slice nz-1 to slice 0: at z = z_last + f dz the periodic map
+    is (1 - f) S_{nz-1}(x, y) + f S_0(x, y), with S_k the bilinear interpolation of slice k (scipy's
+    RegularGridInterpolator on the 2-D slice).  The same holds below z0 (z0 - f dz wraps to weight
+    1 - f) and k periods away, and the wrap is about z0, not about 0.  Random node values, so no
+    smooth structure hides a wrong slice."""
+    rng = np.random.default_rng(17)
+    nz, dz = 12, 1e-3
+    grid = {"x": np.linspace(-3e-3, 3e-3, 6), "y": np.linspace(-2e-3, 4e-3, 5), "z": z0 + np.arange(nz) * dz}
+    vals = {c: rng.standard_normal((6, 5, nz)) for c in "xyz"}
+    f = OPeriodicZ(grid, vals)
+    L = nz * dz
+    M = 3000
+    x, y = rng.uniform(-3e-3, 3e-3, M), rng.uniform(-2e-3, 4e-3, M)
+    fr = rng.uniform(0.0, 1.0, M)
+    xy = np.column_stack([x, y])
+    want = np.column_stack([
+        (1.0 - fr) * RegularGridInterpolator((grid["x"], grid["y"]), vals[c][:, :, -1])(xy)
+        + fr * RegularGridInterpolator((grid["x"], grid["y"]), vals[c][:, :, 0])(xy) for c in "xyz"])
+    z_last = grid["z"][-1]
+    for shift in (0, 1, -2, 9):
+        got = f(np.column_stack([x, y, z_last + fr * dz + shift * L]))
+        assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want))
+    below = f(np.column_stack([x, y, z0 - (1.0 - fr) * dz]))
+    assert np.max(np.abs(below - want)) < 1e-9 * np.max(np.abs(want))
+    # node planes: z = z0 + k L is slice 0 exactly, z = z_last + k L is slice nz-1
+    for k in (0, 3, -1):
+        at0 = f(np.column_stack([x, y, np.full(M, z0 + k * L)]))
+        w0 = np.column_stack([RegularGridInterpolator((grid["x"], grid["y"]), vals[c][:, :, 0])(xy) for c in "xyz"])
+        assert np.max(np.abs(at0 - w0)) < 1e-9 * np.max(np.abs(w0))
