@@ -222,6 +222,14 @@ int pp_scale_offset_batch(const double* a, double scale, int32_t has_scale, doub
 
 /* ---- batched push of many independent bunches (extension: the reference
         tracks one particle array per call) ------------------------------------ */
+/* Staggering flags of a bunch (pp_bunch.flags), for the staggered algorithms BORIS,
+   BORIS_REL and VAY: Pusher.track_batch's half-step velocity kicks (pusher.py:946-958,
+   992-1007).  KICK_START: in the launch whose window starts at the bunch's step 0,
+   kick the velocity by -dt/2 before the first step.  KICK_END: in the launch whose
+   window reaches the bunch's last step, kick it by +dt/2 after that step.  A bunch
+   with both flags ends with the velocity track_batch returns in v_hist[-1]. */
+enum pp_bunch_flags { PP_BUNCH_KICK_START = 1, PP_BUNCH_KICK_END = 2 };
+
 /* One bunch: particles [offset, offset+count) of the shared SoA state. */
 typedef struct pp_bunch {
     int64_t offset;
@@ -229,7 +237,7 @@ typedef struct pp_bunch {
     int32_t algorithm;       /* BORIS, VAY, VAY_REL, BORIS_REL, RK4, RK4_REL or LEAPFROG */
     int32_t nsteps;          /* total steps of this bunch */
     int32_t map_index;       /* which of the launch's maps (0..nmaps-1) */
-    int32_t _pad;
+    int32_t flags;           /* enum pp_bunch_flags (ignored by the other algorithms) */
     double dt;
     double q_over_m;
     double escale;           /* multiplies the map's E components (0..2) */

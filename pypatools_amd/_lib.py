@@ -55,12 +55,13 @@ class PPPushArgs(ctypes.Structure):
 class PPBunch(ctypes.Structure):
     _fields_ = [
         ("offset", c_int64), ("count", c_int64), ("algorithm", c_int32), ("nsteps", c_int32),
-        ("map_index", c_int32), ("_pad", c_int32), ("dt", ctypes.c_double), ("q_over_m", ctypes.c_double),
+        ("map_index", c_int32), ("flags", c_int32), ("dt", ctypes.c_double), ("q_over_m", ctypes.c_double),
         ("escale", ctypes.c_double), ("bscale", ctypes.c_double),
     ]
 
 
 MAX_BATCH_MAPS = 4
+BUNCH_KICK_START, BUNCH_KICK_END = 1, 2          # pp_bunch_flags
 MOMENT_SUMS, MOMENT_CENTRAL = 18, 11          # include/pypat_hip.h PP_MOMENT_SUMS, PP_MOMENT_CENTRAL
 
 

@@ -10,8 +10,13 @@ bunch index in its high bits, so bunches keep their slot ranges).  The window
 loop can be captured once as a HIP graph and replayed: the launch reads its
 step window from a device counter that the graph advances.
 
-Results are per-bunch final positions and velocities (v, also for u-state
-algorithms) in each bunch's original particle order.
+Each bunch follows Pusher.track_batch (src/PyPATools/pusher.py:894-1012) with
+a ScaledField scale on E and B (field.py:661-690): the staggered algorithms
+(boris, boris_rel, vay) take the half-step kick back before their first step
+and the half-step kick forward after their last, inside the batched launches
+(pp_bunch_flags).  Results are per-bunch final positions and velocities (v,
+also for u-state algorithms) in each bunch's original particle order: the
+last records r_hist[-1], v_hist[-1] of track_batch.
 """
 import ctypes
 import time
@@ -20,7 +25,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Stream, synchronize
-from .engine import DEFAULT_REFINE_BITS, U_STATE, _fused_map, motion_axis
+from .engine import DEFAULT_REFINE_BITS, STAGGERED, U_STATE, _fused_map, motion_axis
 from .interpolators import DeviceFieldMap
 
 BATCH_ALGORITHMS = ("boris", "vay", "vay_rel", "boris_rel", "rk4", "rk4_rel", "leapfrog")
@@ -94,6 +99,7 @@ class BatchedTracker:
             t.algorithm = _lib.ALG_IDS[b.algorithm]
             t.nsteps = b.nsteps
             t.map_index = b.map_index
+            t.flags = (_lib.BUNCH_KICK_START | _lib.BUNCH_KICK_END) if b.algorithm in STAGGERED else 0
             t.dt, t.q_over_m, t.escale, t.bscale = b.dt, b.q_over_m, b.escale, b.bscale
         self.offsets, self.counts = offsets, counts
         self.table_host = table
@@ -168,6 +174,10 @@ class BatchedTracker:
         end = self.max_steps if nsteps is None else min(self.max_steps, self.step + int(nsteps))
         spl = self.steps_per_launch
         t0 = time.perf_counter()
+        if self.max_steps == 0 and self.step == 0:
+            # only 0-step bunches: one window [0, 1) gives the staggered ones both half kicks
+            _lib.call("pp_push_batched", ctypes.byref(self._args(0, 1)), stream)
+            self.step = -1
         if self.use_graph:
             self._run_graph(end)
         else:

@@ -708,9 +708,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // per block, so the branches below are scalar).
 // ---------------------------------------------------------------------------
 
+// Half-step velocity kick of a staggered bunch (the kick_only launches of the single-bunch path).
+template <int ALG, bool MIXED, bool PERIODIC>
+DEV d3 batch_kick(int64_t i, d3 r, d3 v, double dtk, const HotMap& m, __amdgpu_buffer_rsrc_t rs,
+                  const AlgConst& ac) {
+    if constexpr (MIXED) return kick_mixed<ALG>(i, r, v, dtk, 0, HotFE32<PERIODIC>{m, rs, hot_scale(m, 1.0), HotRF{}}, ac);
+    else return kick64<ALG>(i, r, v, dtk, 0, HotFE64<PERIODIC>{m, rs, HotRF{}}, ac);
+}
+
+// ns steps from step0; kick_start / kick_end: the staggering half kicks before the first and after
+// the last step (pp_bunch_flags), for the staggered algorithms only.
 template <int ALG, bool MIXED, bool PERIODIC>
 DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double dt, const HotMap& m,
-                     __amdgpu_buffer_rsrc_t rs, const AlgConst& ac) {
+                     __amdgpu_buffer_rsrc_t rs, const AlgConst& ac, bool kick_start, bool kick_end) {
+    constexpr bool STAGGERED = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY;
+    if constexpr (STAGGERED) {
+        if (kick_start) v = batch_kick<ALG, MIXED, PERIODIC>(i, r, v, -0.5 * dt, m, rs, ac);
+    }
     d3 vrec = v;
     const float h = (float)(0.5 * ac.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
@@ -718,6 +732,9 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
     for (int64_t s = 0; s < ns; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs, HotRF{}}, ac);
+    }
+    if constexpr (STAGGERED) {
+        if (kick_end) v = batch_kick<ALG, MIXED, PERIODIC>(i, r, v, 0.5 * dt, m, rs, ac);
     }
 }
 
@@ -733,8 +750,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (i < bd.offset || i >= bd.offset + bd.count || i >= p.n) return;
     const int64_t step0 = p.step0_dev ? *p.step0_dev : p.step0;
     const int64_t end = bd.nsteps < step0 + p.nsteps ? (int64_t)bd.nsteps : step0 + p.nsteps;
-    if (end <= step0) return;
-    const int64_t ns = end - step0;
+    const int64_t ns = end > step0 ? end - step0 : 0;
+    // staggering kicks: in the window holding the bunch's step 0, and in the one reaching its last step
+    // (a bunch of 0 steps takes both in the first window, as track_batch does with nsteps = 0)
+    const bool kick_start = (bd.flags & PP_BUNCH_KICK_START) && step0 == 0;
+    const bool kick_end = (bd.flags & PP_BUNCH_KICK_END) && (ns > 0 ? end == bd.nsteps : step0 == 0 && bd.nsteps == 0);
+    if (ns == 0 && !kick_start && !kick_end) return;
     HotMap m = p.maps[bd.map_index];
     m.sE *= bd.escale;
     m.sB *= bd.bscale;
@@ -752,20 +773,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     d3 r = mk3(p.x[i], p.y[i], p.z[i]);
     d3 v = mk3(p.vx[i], p.vy[i], p.vz[i]);
     switch (bd.algorithm) {
-        case PP_ALG_BORIS: batch_steps<PP_ALG_BORIS, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac); break;
-        case PP_ALG_VAY: batch_steps<PP_ALG_VAY, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac); break;
+        case PP_ALG_BORIS: batch_steps<PP_ALG_BORIS, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end); break;
+        case PP_ALG_VAY: batch_steps<PP_ALG_VAY, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end); break;
         case PP_ALG_VAY_REL:
-            batch_steps<PP_ALG_VAY_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            batch_steps<PP_ALG_VAY_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
             break;
         case PP_ALG_BORIS_REL:
-            batch_steps<PP_ALG_BORIS_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            batch_steps<PP_ALG_BORIS_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
             break;
-        case PP_ALG_RK4: batch_steps<PP_ALG_RK4, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac); break;
+        case PP_ALG_RK4: batch_steps<PP_ALG_RK4, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end); break;
         case PP_ALG_RK4_REL:
-            batch_steps<PP_ALG_RK4_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            batch_steps<PP_ALG_RK4_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
             break;
         case PP_ALG_LEAPFROG:
-            batch_steps<PP_ALG_LEAPFROG, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac);
+            batch_steps<PP_ALG_LEAPFROG, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
             break;
         default: return;
     }

@@ -28,7 +28,7 @@ import numpy as np
 from . import _lib
 from .batch import BATCH_ALGORITHMS, make_map
 from .device import DeviceArray, mem_info, synchronize
-from .engine import DEFAULT_REFINE_BITS, U_STATE, ParticleState, motion_axis
+from .engine import DEFAULT_REFINE_BITS, STAGGERED, U_STATE, ParticleState, motion_axis
 from .interpolators import DeviceFieldMap
 
 
@@ -255,6 +255,9 @@ class TrackingServer:
             t.algorithm = _lib.ALG_IDS[job.algorithm]
             t.nsteps = min(self.chunk_steps, job.nsteps - job.done_steps)
             t.map_index = job.map_index
+            if job.algorithm in STAGGERED:     # track_batch's half kicks in the first and last chunk
+                t.flags = ((_lib.BUNCH_KICK_START if job.done_steps == 0 else 0)
+                           | (_lib.BUNCH_KICK_END if job.done_steps + t.nsteps == job.nsteps else 0))
             t.dt, t.q_over_m, t.escale, t.bscale = job.dt, job.q_over_m, job.escale, job.bscale
             work += job.n * t.nsteps
         nt = ctypes.c_int64()

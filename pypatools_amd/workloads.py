@@ -8,9 +8,11 @@ C2  128^3 map at 1 mm spacing, x, y in [-63.5, 63.5] mm, z in [0, 127] mm,
     Gaussian bunch (sigma = 2 mm in x, y, z) at the map centre, |p| =
     p(2.5 MeV) (1 + 1e-3 N(0,1)) along +z; dt = 5e-11 s, 1e4 steps.
 C3  96 x 96 x 256 TM010-like map (E0 = 10 MV/m, f = 100 MHz, cos/sin(w t + phi),
-    phi from the bunch seed), x, y in [-9.5, 9.5] mm (0.2 mm), z in [0, 255] mm
-    (1 mm); 4e6 electrons at 100 keV along +z in a uniform disk of radius 3 mm
-    at z = 10 mm; dt = 2e-13 s, 5000 steps; 'rk4_rel'.
+    phi from the bunch seed), x, y in [-9.5, 9.5] mm (0.2 mm), z in [0, 318.75] mm
+    (1.25 mm: longer than c N dt = 300 mm, so no electron leaves the map along z
+    in the 5000 steps; they reach about 1.4 MeV and z = 260 mm); 4e6 electrons at
+    100 keV along +z in a uniform disk of radius 3 mm at z = 10 mm; dt = 2e-13 s,
+    5000 steps; 'rk4_rel'.
 All particle inputs are float64 (M, 3) arrays in SI units, drawn with
 numpy.random.default_rng(seed).
 """
@@ -70,7 +72,7 @@ def c2(M=1_000_000, seed=2, n=128):
             "species": "proton", "periodic_z": True}
 
 
-def c3_map(nx=96, nz=256, dxy=0.2e-3, dz=1e-3, E0=1e7, f=1e8):
+def c3_map(nx=96, nz=256, dxy=0.2e-3, dz=1.25e-3, E0=1e7, f=1e8):
     from scipy.special import j0, j1
     x = (np.arange(nx) - (nx - 1) / 2.0) * dxy
     z = np.arange(nz) * dz

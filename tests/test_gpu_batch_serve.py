@@ -53,18 +53,12 @@ def _bunches(rng, n_list, algs):
 
 
 def _alone(b, grid, fields, prec):
+    """The bunch through Tracker.track (track_batch semantics, one record at the end)."""
     E, B = fields
     fe = pa.Field.from_arrays(grid, E, periodic_z=True, scaling=b.escale)
     fb = pa.Field.from_arrays(grid, B, periodic_z=True, scaling=b.bscale)
-    st = ParticleState.from_host(b.r0, b.v0)
-    u = b.algorithm in ("vay", "boris_rel")
-    if u:
-        st.v_to_u()
-    Tracker(b.algorithm, prec, b.q_over_m).run_resident(st, FieldPair(fe, fb), b.nsteps, b.dt)
-    st.unsort()
-    if u:
-        st.u_to_v()
-    return st.to_host()
+    rh, vh, _ = Tracker(b.algorithm, prec, b.q_over_m).track(b.r0, b.v0, fe, fb, b.nsteps, b.dt, b.nsteps)
+    return rh[-1], vh[-1]
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32_mixed", 1e-6), ("fp64_fast", 1e-12)])
