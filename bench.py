@@ -30,7 +30,38 @@ sys.path.insert(0, ROOT)
 GATHER_BYTES = {"fp32_mixed": 192, "fp64_fast": 384, "fp64": 384}   # 8 corners x 6 comps x sizeof(map element)
 FLOPS_PER_STEP = 55 + 135                                          # non-rel Boris push + one 6-comp gather
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 78.6       # MI355X vector peak without packed FP32 (= FP64 vector): 256 CUs x 128 flop/clk x 2.4 GHz
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4        # wave64 VALU instructions per second, whole chip
+# sources of the hot kernel; counters in profiles/pmc_counters.json are used only for this exact source
+KERNEL_SOURCES = ("pypatools_amd/csrc/pp_fast.hip", "pypatools_amd/csrc/pp_internal.h",
+                  "pypatools_amd/csrc/pp_fp64.inc", "pypatools_amd/csrc/pp_bvh.inc")
+
+
+def kernel_source_digest():
+    """sha256 over the hot kernel's source files (the build's inputs for pp_fast.hip)."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def load_counters(precision):
+    """Counters of k_hot for `precision` from profiles/pmc_counters.json, if they were taken on this source.
+    Returns (record or None, status)."""
+    cfile = os.path.join(ROOT, "profiles", "pmc_counters.json")
+    if not os.path.exists(cfile):
+        return None, "no profiles/pmc_counters.json"
+    with open(cfile) as f:
+        data = json.load(f)
+    rec = data.get(precision)
+    if rec is None:
+        return None, f"no counters for {precision}"
+    digest = kernel_source_digest()
+    if rec.get("source_sha256") != digest:
+        return None, f"stale: counters taken on source {str(rec.get('source_sha256'))[:12]}, kernel source is {digest[:12]}"
+    return rec, "counters of this source"
 
 
 def parse():
@@ -43,6 +74,8 @@ def parse():
     ap.add_argument("--nsteps", type=int, default=10_000)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="C2 line only: no loss, space-charge, C3/C4/C5 legs (profiling runs)")
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--sort-every", type=int, default=500, help="steps between cell sorts (0: never)")
     ap.add_argument("--refine-bits", type=int, default=0, help="sub-cell sort bins along the motion (2^bits)")
@@ -59,7 +92,101 @@ def parse():
     ap.add_argument("--pool", type=int, default=60_000_000, help="C5 particle pool (HBM budget)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the AoS->SoA copy kernel over this many MiB (counter calibration) and exit")
+    ap.add_argument("--summarize-pmc", action="store_true",
+                    help="summarise rocprofv3 --pmc CSV output (--pmc-dirs, --copy-dirs) instead of running")
+    ap.add_argument("--pmc-dirs", nargs="*", default=[])
+    ap.add_argument("--copy-dirs", nargs="*", default=[])
+    ap.add_argument("--kernel-regex", default=r"k_hot<1, true, true, false>")
+    ap.add_argument("--pmc-steps-per-launch", type=int, default=500)
+    ap.add_argument("--write-counters", action="store_true",
+                    help="with --summarize-pmc: store the record in profiles/pmc_counters.json")
     return ap.parse_args()
+
+
+def read_pmc(dirs, kernel_regex):
+    """Per-dispatch counter values of the kernels matching `kernel_regex` from rocprofv3 --pmc CSV output
+    (*counter_collection.csv under each dir).  Returns {counter: mean per dispatch}, dispatch count,
+    the matched kernel names and the mean dispatch duration in ns (when the CSV has timestamps)."""
+    import csv
+    import glob
+    import re
+    rx = re.compile(kernel_regex)
+    per = {}            # (file, dispatch) -> {counter: value}
+    dur = {}
+    names = set()
+    for d in dirs:
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn, newline="") as f:
+                for row in csv.DictReader(f):
+                    name = row.get("Kernel_Name", "")
+                    if not rx.search(name):
+                        continue
+                    names.add(name)
+                    key = (fn, row.get("Dispatch_Id") or row.get("Correlation_Id"))
+                    c = per.setdefault(key, {})
+                    c[row["Counter_Name"]] = c.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+                    if row.get("Start_Timestamp") and row.get("End_Timestamp"):
+                        dur[key] = float(row["End_Timestamp"]) - float(row["Start_Timestamp"])
+    if not per:
+        return {}, 0, sorted(names), None
+    counters = sorted({k for c in per.values() for k in c})
+    mean = {k: float(np.mean([c[k] for c in per.values() if k in c])) for k in counters}
+    return mean, len(per), sorted(names), (float(np.mean(list(dur.values()))) if dur else None)
+
+
+def summarize_pmc(args):
+    """Summaries of rocprofv3 --pmc runs: per-dispatch counter averages of the hot kernel, the HBM traffic
+    scaled by the copy probe's factors, and (--write-counters) the record bench.py reads for `traffic` and
+    the VALU issue fraction, stamped with the kernel source digest."""
+    out = {"kernel_regex": args.kernel_regex, "source_sha256": kernel_source_digest()}
+    res = {}
+    for tag, dirs in (("kernel", args.pmc_dirs), ("copy", args.copy_dirs)):
+        if not dirs:
+            continue
+        rx = args.kernel_regex if tag == "kernel" else r"k_util"
+        mean, n, names, dur_ns = read_pmc(dirs, rx)
+        res[tag] = mean
+        out[tag] = {"dispatches": n, "kernels": names, "mean_per_dispatch": mean, "mean_duration_ns": dur_ns}
+    k = res.get("kernel", {})
+    c = res.get("copy", {})
+    copy_bytes = args.copy_probe_mib * 1024 * 1024 // 24 * 24
+    if c and k and "FETCH_SIZE" in k and "WRITE_SIZE" in k:
+        f_fetch = c["FETCH_SIZE"] * 1024 / copy_bytes
+        f_write = c["WRITE_SIZE"] * 1024 / copy_bytes
+        hbm = (k["FETCH_SIZE"] / f_fetch + k["WRITE_SIZE"] / f_write) * 1024
+        out["hbm"] = {"fetch_factor": f_fetch, "write_factor": f_write, "bytes_per_dispatch": hbm}
+    units = args.particles * args.pmc_steps_per_launch
+    waves = -(-args.particles // 64)
+    if "SQ_INSTS_VALU" in k:
+        out["per_wavefront_step"] = {name: k[name] / (waves * args.pmc_steps_per_launch)
+                                     for name in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD",
+                                                  "SQ_INSTS_SMEM") if name in k}
+    print(json.dumps(out, indent=1), flush=True)
+    if args.write_counters:
+        cfile = os.path.join(ROOT, "profiles", "pmc_counters.json")
+        data = {}
+        if os.path.exists(cfile):
+            with open(cfile) as f:
+                data = json.load(f)
+        rec = data.get(args.precision, {})
+        if rec.get("source_sha256") != out["source_sha256"]:
+            rec = {}
+        rec.update({"kernel": out["kernel"]["kernels"][0] if out.get("kernel", {}).get("kernels") else None,
+                    "workload": f"C2 Boris {args.precision}, {args.particles} particles",
+                    "particles": args.particles, "steps_per_launch": args.pmc_steps_per_launch,
+                    "source_sha256": out["source_sha256"]})
+        if "hbm" in out:
+            rec["hbm_bytes_per_launch"] = out["hbm"]["bytes_per_dispatch"]
+            rec["hbm_bytes_per_particle_step"] = out["hbm"]["bytes_per_dispatch"] / units
+        for name, key in (("SQ_INSTS_VALU", "valu_insts_per_wavefront_step"),
+                          ("SQ_INSTS_LDS", "lds_insts_per_wavefront_step"),
+                          ("SQ_INSTS_VMEM_RD", "vmem_insts_per_wavefront_step"),
+                          ("SQ_INSTS_SALU", "salu_insts_per_wavefront_step")):
+            if name in out.get("per_wavefront_step", {}):
+                rec[key] = out["per_wavefront_step"][name]
+        data[args.precision] = rec
+        with open(cfile, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
 
 
 def copy_probe(mib):
@@ -259,8 +386,9 @@ def cpu_baseline(w, seconds=15.0):
                       f"(vectorised over particles, single thread), {el:.1f} s"}
 
 
-def bench_c4(args):
-    """C4: 512 bunches (~4e7 protons/H2+/electrons), one batched launch per window, autotuned."""
+def run_c4(args, reps=None, warmup=None):
+    """C4: 512 bunches (~4e7 protons/H2+/electrons), one batched launch per window, autotuned.
+    Returns (value, bunches, tracker, extra, cfg)."""
     import pypatools_amd as pa
     from pypatools_amd.autotune import autotune_batch
     from pypatools_amd.batch import BatchedTracker, c4_bunches
@@ -281,18 +409,26 @@ def bench_c4(args):
         cfg = {"block_size": 256, "steps_per_launch": 500, "sort_every": 500}
     bt = BatchedTracker(fields, args.precision, use_graph=args.graph, **cfg)
     rates = []
-    for rep in range(args.warmup + args.steps):
+    reps = args.steps if reps is None else reps
+    warmup = args.warmup if warmup is None else warmup
+    for rep in range(warmup + reps):
         bt.load(bunches)
         synchronize()
         t = time.perf_counter()
         bt.run()
         el = time.perf_counter() - t
-        if rep >= args.warmup:
+        if rep >= warmup:
             rates.append(bt.particle_steps / el)
     res = bt.results()
     r_all = np.concatenate([r for r, _ in res])
     inside = float(np.mean((np.abs(r_all[:, 0]) <= 0.0635) & (np.abs(r_all[:, 1]) <= 0.0635)))
     value = float(np.median(rates))
+    extra = {"generate_s": gen_s, "in_map_fraction_end": inside, "rates": rates, **tune}
+    return value, bunches, bt, extra, cfg
+
+
+def bench_c4(args):
+    value, bunches, bt, extra, cfg = run_c4(args)
     out = {"metric": "particle-steps/sec on 1 MI355X (C4 batched persistent launch)", "value": value,
            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * bt.particle_steps / value, "higher_is_better": True, "scaling": "weak",
@@ -302,7 +438,7 @@ def bench_c4(args):
                                   f"{bt.particle_steps:.3e} particle-steps, shared 128^3 map, per-bunch scale "
                                   "0.5-1.5, species proton/H2+/electron, Boris or Vay, 1e3-1e4 steps",
                       "precision": args.precision, "hip_graph": bool(args.graph), **cfg},
-           "extra": {"generate_s": gen_s, "in_map_fraction_end": inside, "rates": rates, **tune}}
+           "extra": extra}
     print(json.dumps(out), flush=True)
 
 
@@ -331,8 +467,9 @@ def _track_rate(tracker, st, fp, nsteps, dt, reps=1, t0=0.0):
     return float(np.median(rates)), last.to_host()
 
 
-def bench_c3(args):
-    """C3: RK4_rel through a TM010-like RF map (E cos, B sin), 4e6 electrons, 5000 steps; FP32-mixed vs FP64."""
+def run_c3(args, reps=None):
+    """C3: RK4_rel through a TM010-like RF map (E cos, B sin), 4e6 electrons, 5000 steps; FP32-mixed vs FP64.
+    Returns (value, extra, M, N, phase)."""
     import pypatools_amd as pa
     from pypatools_amd.engine import FieldPair, ParticleState, Tracker
     from pypatools_amd.workloads import c3_bunch, c3_map
@@ -348,7 +485,8 @@ def bench_c3(args):
     res = {}
     for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
         tr = Tracker("rk4_rel", prec, qm, sort_every=args.sort_every, refine_bits=args.refine_bits)
-        res[prec] = _track_rate(tr, st0, fp, N, dt, reps=args.steps if prec == args.precision else 1)
+        res[prec] = _track_rate(tr, st0, fp, N, dt, reps=(args.steps if reps is None else reps)
+                                if prec == args.precision else 1)
     value, (rm, vm) = res[args.precision]
     extra = {}
     if "fp64" in res:
@@ -360,6 +498,16 @@ def bench_c3(args):
                  "pos_err_max_m": float(np.max(dr)), "energy_err_rel_max": float(np.max(np.abs(gm - g64) / (g64 - 1))),
                  "speedup_over_fp64": value / res["fp64"][0]}
     extra["mean_kinetic_gain_keV"] = float(np.mean(_gamma(vm) - _gamma(v0)) * 510.99895069)
+    # inside the map at the end: |x|, |y| within the transverse half-width and z within [z0, z_last]
+    half = float(grid["x"][-1])
+    extra["in_map_fraction_end"] = float(np.mean((np.abs(rm[:, 0]) <= half) & (np.abs(rm[:, 1]) <= half)
+                                                 & (rm[:, 2] >= grid["z"][0]) & (rm[:, 2] <= grid["z"][-1])))
+    extra["z_end_range_m"] = [float(rm[:, 2].min()), float(rm[:, 2].max())]
+    return value, extra, M, N, phase
+
+
+def bench_c3(args):
+    value, extra, M, N, phase = run_c3(args)
     out = {"metric": "particle-steps/sec on 1 MI355X (C3 RK4_rel through an RF map)", "value": value,
            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
            "ms_per_step": 1e3 * M * N / value, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -403,8 +551,9 @@ def bench_c1(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_c5(args):
-    """C5: Poisson arrivals, continuous batching, p50/p99 latency at a load set from measured capacity."""
+def run_c5(args, n_jobs=None):
+    """C5: Poisson arrivals, continuous batching, p50/p99 latency at a load set from measured capacity.
+    Returns (jobs, lam, capacity, measured, span, srv)."""
     import pypatools_amd as pa
     from pypatools_amd.serving import TrackingServer, c5_jobs
     from pypatools_amd.workloads import c2_map
@@ -433,7 +582,7 @@ def bench_c5(args):
         srv.particle_steps = 0
         srv.rounds = 0
     lam = args.load * capacity / mean_work
-    jobs = c5_jobs(args.jobs, lam)
+    jobs = c5_jobs(args.jobs if n_jobs is None else n_jobs, lam)
     t0 = time.perf_counter()
     for j in jobs:
         j.arrival += t0
@@ -447,10 +596,25 @@ def bench_c5(args):
             continue
         srv.round(now)
     span = time.perf_counter() - t0
+    return jobs, lam, capacity, measured, span, srv
+
+
+def c5_summary(jobs, lam, capacity, measured, span, srv, load):
     lat = np.array([j.latency for j in jobs])
     work = sum(j.n * j.nsteps for j in jobs)
+    return {"jobs": len(jobs), "offered_load": load, "arrival_rate_jobs_per_s": lam,
+            "capacity_particle_steps_per_s": capacity, "capacity_measured": measured is not None,
+            "p50_s": float(np.percentile(lat, 50)), "p90_s": float(np.percentile(lat, 90)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+            "throughput_particle_steps_per_s": work / span, "makespan_s": span, "rounds": srv.rounds,
+            "pool_particles": srv.pool_n}
+
+
+def bench_c5(args):
+    jobs, lam, capacity, measured, span, srv = run_c5(args)
+    summ = c5_summary(jobs, lam, capacity, measured, span, srv, args.load)
     out = {"metric": "p99 job latency, continuous-batching tracking server on 1 MI355X (C5)",
-           "value": float(np.percentile(lat, 99)), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "value": summ["p99_s"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": span * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32+f64" if args.precision == "fp32_mixed" else "f64", "data": "synthetic",
            "config": {"workload": f"C5: {len(jobs)} jobs, Poisson arrivals at {lam:.2f} jobs/s "
@@ -458,11 +622,7 @@ def bench_c5(args):
                                   f"{'measured ' if measured else ''}capacity), N log-uniform "
                                   "1e3-2e6, steps log-uniform 1e2-2e4, Boris/Vay/RK4, 4 resident 128^3 maps, "
                                   "256-step chunks", "precision": args.precision, "pool_particles": srv.pool_n},
-           "extra": {"p50_s": float(np.percentile(lat, 50)), "p90_s": float(np.percentile(lat, 90)),
-                     "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
-                     "throughput_particle_steps_per_s": work / span, "makespan_s": span, "rounds": srv.rounds,
-                     "arrival_rate_jobs_per_s": lam, "capacity_particle_steps_per_s": capacity,
-                     "capacity_measured": measured is not None}}
+           "extra": summ}
     print(json.dumps(out), flush=True)
 
 
@@ -511,6 +671,11 @@ def bench_spacecharge(args):
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        if not args.copy_probe_mib:
+            args.copy_probe_mib = 768
+        summarize_pmc(args)
+        return
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
         return
@@ -585,6 +750,7 @@ def main():
                 dr = np.linalg.norm(r_mixed - r64, axis=1)
                 extra["pos_err_rel_path_max"] = float(np.max(dr / path))
                 extra["pos_err_max_m"] = float(np.max(dr))
+                extra["pos_err_denominator"] = "|v0| N dt (path length); FP64 strict result = oracle bit for bit"
             del s64
 
     # ---- warmup + timed bench steps (each restarts the track; inputs resident in HBM)
@@ -613,28 +779,31 @@ def main():
     avg_ms = float(np.mean(kern_ms))
     work = args.steps * M * N
     value = work / el
-    # roofline of the dominant kernel (k_hot<boris>): algorithmic gather bytes per launch / launch time
+    for prec in ("fp64_fast", "fp64"):
+        if f"{prec}_particle_steps_per_s" in extra:
+            extra[f"speedup_over_{prec}"] = value / extra[f"{prec}_particle_steps_per_s"]
+    # roofline of the dominant kernel (k_hot<boris>): the path has no dense contraction and its gathers are
+    # served from LDS/L2 (HBM sees a fraction of a byte per particle-step), so the bound is the vector ALU:
+    # algorithmic flops per launch (190 per particle-step) / launch time, against the 78.6 TFLOP/s vector peak
     units = M * steps_per_launch
-    achieved = GATHER_BYTES[args.precision] * units / (avg_ms * 1e-3) / 1e9
+    avg_s = avg_ms * 1e-3
+    tflops = FLOPS_PER_STEP * units / avg_s / 1e12
+    gather_gbs = GATHER_BYTES[args.precision] * units / avg_s / 1e9
     probe = gather_probe(fe, fb, M, steps_per_launch)
     gpeak = probe["wave_uniform_cells_GBps"]
-    # counters of the same kernel and workload from the committed rocprofv3 runs: HBM bytes per
-    # launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and VALU instructions per wavefront-step
-    traffic, valu = None, None
-    cfile = os.path.join(ROOT, "profiles", "pmc_counters.json")
-    if os.path.exists(cfile):
-        with open(cfile) as f:
-            rec = json.load(f).get(args.precision)
-        if rec and rec["steps_per_launch"] == steps_per_launch:
+    # counters of the same kernel and workload from the committed rocprofv3 runs on this source: HBM bytes
+    # per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and VALU instructions per wavefront-step
+    rec, cstatus = load_counters(args.precision)
+    traffic, issue = None, {"status": cstatus}
+    if rec is not None:
+        if rec["steps_per_launch"] == steps_per_launch and rec.get("particles") == M:
             traffic = rec["hbm_bytes_per_launch"]
-        if rec:
-            wave_steps = -(-M // 64) * steps_per_launch
-            issue = VALU_ISSUE_PER_S * avg_ms * 1e-3                # wave64 VALU issue slots in one launch
-            valu = {"kind": "valu", "insts_per_wavefront_step": rec["valu_insts_per_wavefront_step"],
-                    "frac": rec["valu_insts_per_wavefront_step"] * wave_steps / issue,
-                    "peak": "256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz "
-                            "(78.6 TFLOP/s FP32 non-packed = FP64 vector peak)",
-                    "note": "instructions per wavefront-step from profiles/pmc_counters.json, launch time live"}
+        wave_steps = -(-M // 64) * steps_per_launch
+        slots = VALU_ISSUE_PER_S * avg_s                     # wave64 VALU issue slots in one launch
+        issue.update({"valu_insts_per_wavefront_step": rec["valu_insts_per_wavefront_step"],
+                      "frac": rec["valu_insts_per_wavefront_step"] * wave_steps / slots,
+                      "slots": "256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz",
+                      "flop_efficiency_per_instruction": FLOPS_PER_STEP / (2 * rec["valu_insts_per_wavefront_step"])})
     out = {
         "metric": "particle-steps/sec on 1 MI355X, FP32-mixed vs FP64 at stated error; p99 job latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -643,17 +812,21 @@ def main():
         "config": {"workload": "C2: Boris + trilinear gather, 128^3 E/B map (1 mm, periodic z), "
                                f"{M} protons 2.5 MeV Gaussian sigma 2 mm, dt 5e-11 s, {N} steps per bench step",
                    "precision": args.precision, "particles": M, "time_steps_per_bench_step": N},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / VALU_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true, false> (boris, "
-                               f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.2f} ms)",
-                     "algorithmic_bytes_per_particle_step": GATHER_BYTES[args.precision],
-                     "gflops": FLOPS_PER_STEP * units / (avg_ms * 1e-3) / 1e9,
-                     "limiting_roofline": valu,
-                     "gather_roofline": {"level": "vector memory path (buffer loads through L1), measured peak; "
-                                                  "the kernel stages cells through LDS and exceeds it",
-                                         "peak": gpeak, "unit": "GB/s", "frac": achieved / gpeak,
-                                         "probe": probe}},
+                               f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.3f} ms by HIP events)",
+                     "flops_per_particle_step": FLOPS_PER_STEP, "particle_steps_per_launch": units,
+                     "peak_note": "vector peak without packed FP32 (not used on this path) = FP64 vector peak",
+                     "valu_issue": issue,
+                     "hbm": {"bytes_per_launch": traffic,
+                             "bytes_per_particle_step": traffic / units if traffic else None,
+                             "GBps": traffic / avg_s / 1e9 if traffic else None,
+                             "frac": traffic / avg_s / 1e9 / HBM_PEAK_GBS if traffic else None},
+                     "gather": {"algorithmic_bytes_per_particle_step": GATHER_BYTES[args.precision],
+                                "GBps": gather_gbs, "vector_path_probe_peak_GBps": gpeak,
+                                "frac_of_probe": gather_gbs / gpeak, "probe": probe,
+                                "note": "cells are staged through LDS, so the gather rate can exceed the probe"}},
         "extra": extra,
     }
     # beam diagnostics on the resident state: the C2 bunch, and a 4e7-particle state (HBM-resident)
@@ -662,10 +835,24 @@ def main():
         arr.fill(val)
     out["extra"]["beam_moments"] = [moments_bandwidth(st0), moments_bandwidth(big)]
     del big
-    out["extra"]["electrode_collision"] = collision_rate(st0, st)
-    out["extra"]["lossy_tracking"] = lossy_tracking(w, fp, qm)
-    out["extra"]["space_charge"] = space_charge_timing(st0)
-    out["extra"]["space_charge_electrodes"] = space_charge_timing(st0, electrodes=True)
+    if not args.no_extras:
+        out["extra"]["electrode_collision"] = collision_rate(st0, st)
+        out["extra"]["lossy_tracking"] = lossy_tracking(w, fp, qm)
+        out["extra"]["space_charge"] = space_charge_timing(st0)
+        out["extra"]["space_charge_electrodes"] = space_charge_timing(st0, electrodes=True)
+        del st, st0
+        # the other workloads of the metric: C3 (RK4_rel + RF), C4 (batched) and C5 (served, p99)
+        t = time.perf_counter()
+        c3v, c3x, c3m, c3n, _ = run_c3(args, reps=1)
+        out["extra"]["c3"] = dict(c3x, particle_steps_per_s=c3v, particles=c3m, steps=c3n, wall_s=time.perf_counter() - t)
+        t = time.perf_counter()
+        c4v, c4b, c4t, c4x, c4cfg = run_c4(args, reps=1, warmup=0)
+        out["extra"]["c4"] = {"particle_steps_per_s": c4v, "bunches": len(c4b), "particles": int(sum(b.n for b in c4b)),
+                              "particle_steps": c4t.particle_steps, "in_map_fraction_end": c4x["in_map_fraction_end"],
+                              "config": c4cfg, "wall_s": time.perf_counter() - t}
+        del c4t, c4b
+        t = time.perf_counter()
+        out["extra"]["c5"] = dict(c5_summary(*run_c5(args), args.load), wall_s=time.perf_counter() - t)
     out["config"]["sort_every"] = args.sort_every
     out["config"]["refine_bits"] = args.refine_bits
     if not args.no_cpu:
