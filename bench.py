@@ -78,7 +78,8 @@ def parse():
                     help="C2 line only: no loss, space-charge, C3/C4/C5 legs (profiling runs)")
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--sort-every", type=int, default=500, help="steps between cell sorts (0: never)")
-    ap.add_argument("--refine-bits", type=int, default=0, help="sub-cell sort bins along the motion (2^bits)")
+    ap.add_argument("--refine-bits", type=int, default=None,
+                    help="sub-cell sort bins along the motion (2^bits); default: the engine's")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0: library default 256)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lossy", "spacecharge"])
     ap.add_argument("--bunches", type=int, default=512)
@@ -100,7 +101,11 @@ def parse():
     ap.add_argument("--pmc-steps-per-launch", type=int, default=500)
     ap.add_argument("--write-counters", action="store_true",
                     help="with --summarize-pmc: store the record in profiles/pmc_counters.json")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.refine_bits is None:
+        from pypatools_amd.engine import DEFAULT_REFINE_BITS
+        args.refine_bits = DEFAULT_REFINE_BITS
+    return args
 
 
 def read_pmc(dirs, kernel_regex):
@@ -381,9 +386,28 @@ def cpu_baseline(w, seconds=15.0):
         r, v = pu.push_batch(r, v, fe, fb, w["dt"])
         steps += 1
     el = time.perf_counter() - t0
+    # the reference's as-shipped no-numba structure (pusher.py:755-760): the fields of all particles
+    # through the vectorised gather, then the Boris kernel particle by particle on (3,) arrays
+    from oracle import kernels as K
+    Ms = 2000
+    r2, v2 = w["r0"][:Ms].copy(), w["v0"][:Ms].copy()
+    qm = OS.q_over_m("proton")
+    steps2 = 0
+    t1 = time.perf_counter()
+    while time.perf_counter() - t1 < seconds / 3:
+        E, B = fe(r2), fb(r2)
+        for i in range(Ms):
+            rn, vn = K.boris_step(r2[i:i + 1], v2[i:i + 1], E[i:i + 1], B[i:i + 1], w["dt"], qm, K.sq_pow)
+            r2[i], v2[i] = rn[0], vn[0]
+        steps2 += 1
+    el2 = time.perf_counter() - t1
     return {"value": M * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": f"C2 Boris + trilinear, {M} protons x {steps} steps, oracle/ canonical NumPy FP64 "
-                      f"(vectorised over particles, single thread), {el:.1f} s"}
+                      f"(vectorised over particles, single thread), {el:.1f} s",
+            "as_shipped": {"value": Ms * steps2 / el2, "unit": "particle-steps/s", "cores": 1,
+                           "sample": f"{Ms} protons x {steps2} steps of the reference's no-numba structure "
+                                     f"(vectorised gather, per-particle Boris kernel, libm pow squares), "
+                                     f"{el2:.1f} s"}}
 
 
 def run_c4(args, reps=None, warmup=None):

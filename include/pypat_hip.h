@@ -65,7 +65,7 @@ enum pp_precision {
 
 enum pp_field_kind { PP_FIELD_ZERO = 0, PP_FIELD_CONST = 1, PP_FIELD_MAP = 2, PP_FIELD_ARRAY = 3 };
 enum pp_rf_mode { PP_RF_NONE = 0, PP_RF_COS = 1, PP_RF_SIN = 2 };
-enum pp_map_flags { PP_MAP_PERIODIC_Z = 1, PP_MAP_NO_FP32 = 2, PP_MAP_NO_COEF = 4 };
+enum pp_map_flags { PP_MAP_PERIODIC_Z = 1, PP_MAP_NO_FP32 = 2, PP_MAP_NO_COEF = 4, PP_MAP_NO_FP64 = 8 };
 
 typedef struct pp_map pp_map;          /* device-resident field map (opaque) */
 typedef void* pp_stream;               /* hipStream_t */
@@ -163,6 +163,18 @@ int pp_event_create(pp_event* e);
 int pp_event_destroy(pp_event e);
 int pp_event_record(pp_event e, pp_stream s);
 int pp_event_elapsed_ms(float* ms, pp_event start, pp_event stop);   /* blocking on stop */
+/* Asynchronous transfers for the serving loop (no counterpart in the reference, whose
+   pushers run on the host).  pp_host_alloc returns page-locked host memory
+   (hipHostMalloc); the *_async copies enqueue on the stream and return at once (host
+   buffers must be page-locked and stay untouched until an event recorded after the copy
+   has completed); pp_event_query sets *done = 1 once the event has completed, without
+   blocking; pp_event_sync blocks until it has. */
+int pp_host_alloc(void** hptr, size_t bytes);
+int pp_host_free(void* hptr);
+int pp_memcpy_h2d_async(void* dst, const void* src, size_t bytes, pp_stream s);
+int pp_memcpy_d2h_async(void* dst, const void* src, size_t bytes, pp_stream s);
+int pp_event_query(pp_event e, int32_t* done);
+int pp_event_sync(pp_event e);
 
 /* ---- field maps (Field.from_arrays + get_interpolator, field.py:345-387,
         interpolators.py:533-610) ------------------------------------------ */
@@ -174,7 +186,10 @@ int pp_event_elapsed_ms(float* ms, pp_event start, pp_event stop);   /* blocking
  * Field._get_field_1d/_2d evaluate them, field.py:457-489).
  * fill_value: value of every component outside the grid (bounds_error=False,
  * interpolators.py:94-97; Field uses 0.0, field.py:380-383).
- * flags: PP_MAP_PERIODIC_Z wraps z with period nz*dz (extension).
+ * flags: PP_MAP_PERIODIC_Z wraps z with period nz*dz (extension).  PP_MAP_NO_FP32 /
+ * PP_MAP_NO_FP64 skip the FP32 (192 B per 6-component cell) / FP64 (384 B) coefficient
+ * table, PP_MAP_NO_COEF both: maps for FP32-mixed pushes only need the FP32 table, and
+ * launches of a precision whose table is missing fail with PP_ERR_UNSUPPORTED.
  * Blocking (uploads and builds the cell-coefficient tables).
  */
 int pp_map_create(const double* values, int32_t ncomp, int32_t nx, int32_t ny, int32_t nz,
@@ -188,6 +203,28 @@ int pp_map_bytes(const pp_map* map, size_t* bytes);
    pts, out: device float64 (M, 3).  rf terms are evaluated with factor rf_value. */
 int pp_field_eval(const pp_field* field, int32_t precision, const double* pts, int64_t m,
                   double rf_value, double* out, pp_stream s);
+
+/* Spline interpolation of gridded values: CoordinateMapper.__call__ with the orders of its
+   methods 'nearest' 0, 'linear' 1, 'cubic' 3, 'quintic' 5 (interpolators.py:450-460, 469-510:
+   np.interp to fractional node indices, then scipy.ndimage.map_coordinates), and the
+   'nearest' method of the 'scipy' backend (RegularGridInterpolator, interpolators.py:585).
+   values: host float64 [ncomp][n0]..[n_{ndim-1}] (last axis fastest); axis0..2: host node
+   coordinates of the first ndim axes, strictly increasing (n >= 2 each for PP_SPLINE_RGI).
+   flags: PP_SPLINE_NEAREST_MODE = map_coordinates mode 'nearest' (the reference's
+   bounds_error=True case: callers reject points outside the grid first); otherwise mode
+   'constant': points outside the grid on any axis get cval.  PP_SPLINE_RGI: order 0 with
+   RegularGridInterpolator's rule (node i+1 only past the cell's midpoint, ties to i).
+   Coefficients of orders 3 and 5 are computed on the device at creation (blocking).
+   pp_spline_eval: pts device, point i's coordinates at pts[i*stride + col0 + a], a < ndim;
+   out device [m][ncomp].  Stream-ordered. */
+typedef struct pp_spline pp_spline;
+enum pp_spline_flags { PP_SPLINE_NEAREST_MODE = 1, PP_SPLINE_RGI = 2 };
+int pp_spline_create(const double* values, int32_t ncomp, int32_t ndim, const int64_t* shape, const double* axis0,
+                     const double* axis1, const double* axis2, int32_t order, int32_t flags, double cval,
+                     pp_spline** out);
+int pp_spline_destroy(pp_spline* s);
+int pp_spline_eval(const pp_spline* s, const double* pts, int64_t m, int32_t stride, int32_t col0, double* out,
+                   pp_stream st);
 
 /* ---- pushers -------------------------------------------------------------- */
 /* Fused multi-step push: Pusher.track_batch step loop (pusher.py:966-990) /

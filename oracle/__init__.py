@@ -52,7 +52,12 @@ kernels.vay_kick_ext                   exact E x B drift; relativistic gyro angl
 interp.interp_nd                       exact on multilinear fields; scipy
                                        RegularGridInterpolator and map_coordinates;
                                        golden Field.__call__ outputs; face / outside points
-interp.interp_coordinate_mapper        golden 'fast' backend outputs (bitwise)
+interp.interp_coordinate_mapper        golden 'fast' backend outputs (bitwise, order 1); orders
+                                       0/3/5: scipy map_coordinates itself, node values
+                                       reproduced, half-way rounding of order 0, fill outside
+                                       (test_oracle_interp.py)
+interp.interp_rgi_nearest              scipy RegularGridInterpolator itself; midpoint ties to
+                                       the lower node (test_oracle_interp.py)
 field.OField / OScaled / OComposite    golden Field.__call__ outputs for every field kind,
                                        backend and the field algebra
 field.OPeriodicZ (extension)           wrap cell = (1 - f) slice[nz-1] + f slice[0] by

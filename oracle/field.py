@@ -21,26 +21,30 @@ from .interp import interp_coordinate_mapper, interp_nd
 
 
 class OField:
-    def __init__(self, dim=0, const=None, points=None, values=None, scaling=1.0, interp="numba"):
+    def __init__(self, dim=0, const=None, points=None, values=None, scaling=1.0, interp="numba", order=1):
         self.dim = dim
         self.const = const
         self.points = points
         self.values = values
         self.scaling = scaling
         # 'numba': NumbaInterpolator semantics (canonical); 'coordmapper': the
-        # CoordinateMapper path the reference takes without numba.
-        self._interp = interp_nd if interp == "numba" else interp_coordinate_mapper
+        # CoordinateMapper path the reference takes without numba, of spline `order`
+        # (Field(method='nearest' | 'linear' | 'cubic' | 'quintic'), interpolators.py:450-455).
+        if interp == "numba":
+            self._interp = interp_nd
+        else:
+            self._interp = lambda p, v, q, fill: interp_coordinate_mapper(p, v, q, fill, order=order)
 
     @classmethod
     def constant(cls, fx, fy, fz, scaling=1.0):
         return cls(dim=0, const={"x": fx, "y": fy, "z": fz}, scaling=scaling)
 
     @classmethod
-    def from_arrays(cls, grid, values, scaling=1.0, interp="numba"):
+    def from_arrays(cls, grid, values, scaling=1.0, interp="numba", order=1):
         """field.py:369-387: dim = number of axes with more than one node."""
         pts = tuple(np.asarray(grid[k], dtype=np.float64) for k in "xyz" if k in grid and len(grid[k]) > 1)
         vals = {c: np.asarray(values[c], dtype=np.float64).reshape(tuple(len(p) for p in pts)) for c in values}
-        return cls(dim=len(pts), points=pts, values=vals, scaling=scaling, interp=interp)
+        return cls(dim=len(pts), points=pts, values=vals, scaling=scaling, interp=interp, order=order)
 
     def __call__(self, pts):
         pts = np.atleast_2d(np.asarray(pts, dtype=np.float64))

@@ -12,8 +12,14 @@ src/PyPATools/field_src/interpolators.py vectorised over points:
 
 ``interp_coordinate_mapper`` restates CoordinateMapper (interpolators.py:415-526):
 np.interp to fractional indices (left=-1, right=n) then
-scipy.ndimage.map_coordinates(order=1, mode='constant', cval=fill).  This is
-the path the reference takes when numba is absent (interpolators.py:567-571).
+scipy.ndimage.map_coordinates(order, mode='constant', cval=fill) -- order 0, 1,
+3 or 5 for the methods 'nearest', 'linear', 'cubic', 'quintic'.  This is the
+path the reference takes when numba is absent (interpolators.py:567-571).  Its
+arithmetic is the library routine itself (scipy 1.15 here); tests pin it further
+by the interpolation property (node values are reproduced), the rounding rule of
+order 0 and the exactness of order 1 on linear data.
+``interp_rgi_nearest`` is the 'scipy' backend's RegularGridInterpolator
+'nearest' (the library routine).
 """
 import numpy as np
 from scipy.ndimage import map_coordinates
@@ -77,11 +83,25 @@ def interp_nd(points, values, xi, fill_value=0.0):
     return np.where(ok, res, float(fill_value))
 
 
-def interp_coordinate_mapper(points, values, xi, fill_value=0.0):
-    """CoordinateMapper.__call__ with bounds_error=False, order=1 (interpolators.py:469-520)."""
+def interp_coordinate_mapper(points, values, xi, fill_value=0.0, order=1, bounds_error=False):
+    """CoordinateMapper.__call__ (interpolators.py:469-520): np.interp to fractional node indices
+    (left=-1, right=n), then map_coordinates of `order` (method 'nearest' 0, 'linear' 1, 'cubic' 3,
+    'quintic' 5, :450-455) in mode 'constant' with cval=fill_value, or mode 'nearest' with cval 0
+    when bounds_error (:504-510; the bounds check itself, :523-526, is the caller's)."""
     xi = np.atleast_2d(np.asarray(xi, dtype=np.float64))
     coords = np.empty((xi.shape[0], len(points)))
     for a, g in enumerate(points):
         g = np.asarray(g)
         coords[:, a] = np.interp(xi[:, a], g, np.arange(len(g)), left=-1, right=len(g))
-    return map_coordinates(np.asarray(values), coords.T, order=1, mode="constant", cval=fill_value)
+    mode = "nearest" if bounds_error else "constant"
+    cval = 0.0 if bounds_error else fill_value
+    return map_coordinates(np.asarray(values, dtype=np.float64), coords.T, order=order, mode=mode, cval=cval)
+
+
+def interp_rgi_nearest(points, values, xi, fill_value=0.0):
+    """The 'scipy' backend's method 'nearest' (interpolators.py:585): RegularGridInterpolator."""
+    from scipy.interpolate import RegularGridInterpolator
+    rgi = RegularGridInterpolator(tuple(np.asarray(p, dtype=np.float64) for p in points),
+                                  np.asarray(values, dtype=np.float64), method="nearest", bounds_error=False,
+                                  fill_value=fill_value)
+    return rgi(np.atleast_2d(np.asarray(xi, dtype=np.float64)))

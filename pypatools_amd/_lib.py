@@ -19,7 +19,7 @@ ALG_IDS = {
 PRECISIONS = {"fp64": 0, "fp64_fast": 1, "fp32_mixed": 2}
 FIELD_ZERO, FIELD_CONST, FIELD_MAP, FIELD_ARRAY = 0, 1, 2, 3
 RF_NONE, RF_COS, RF_SIN = 0, 1, 2
-MAP_PERIODIC_Z, MAP_NO_FP32, MAP_NO_COEF = 1, 2, 4
+MAP_PERIODIC_Z, MAP_NO_FP32, MAP_NO_COEF, MAP_NO_FP64 = 1, 2, 4, 8
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int64 = ctypes.c_int64
@@ -60,6 +60,7 @@ class PPBunch(ctypes.Structure):
     ]
 
 
+SPLINE_NEAREST_MODE, SPLINE_RGI = 1, 2          # pp_spline_flags
 MAX_BATCH_MAPS = 4
 BUNCH_KICK_START, BUNCH_KICK_END = 1, 2          # pp_bunch_flags
 MOMENT_SUMS, MOMENT_CENTRAL = 18, 11          # include/pypat_hip.h PP_MOMENT_SUMS, PP_MOMENT_CENTRAL
@@ -111,6 +112,16 @@ SIGNATURES = {
     "pp_event_destroy": (c_int32, [c_void_p]),
     "pp_event_record": (c_int32, [c_void_p, c_void_p]),
     "pp_event_elapsed_ms": (c_int32, [ctypes.POINTER(ctypes.c_float), c_void_p, c_void_p]),
+    "pp_spline_create": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                   c_int32, ctypes.c_double, ctypes.POINTER(c_void_p)]),
+    "pp_spline_destroy": (c_int32, [c_void_p]),
+    "pp_spline_eval": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p]),
+    "pp_host_alloc": (c_int32, [ctypes.POINTER(c_void_p), ctypes.c_size_t]),
+    "pp_host_free": (c_int32, [c_void_p]),
+    "pp_memcpy_h2d_async": (c_int32, [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "pp_memcpy_d2h_async": (c_int32, [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "pp_event_query": (c_int32, [c_void_p, ctypes.POINTER(c_int32)]),
+    "pp_event_sync": (c_int32, [c_void_p]),
     "pp_map_create": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,
                                 ctypes.c_double, c_int32, ctypes.POINTER(c_void_p)]),
     "pp_map_destroy": (c_int32, [c_void_p]),

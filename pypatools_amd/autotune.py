@@ -3,8 +3,10 @@
 Coordinate search over block size, steps per launch and cell-sort cadence, each
 candidate timed on a short sample of the real workload (the same bunches, a
 few hundred steps, state re-uploaded for every candidate).  Results are cached
-in a JSON file keyed by a workload signature (device, precision, particle
-count, bunch count, map shape) so a serving process tunes once.
+per process, keyed by a workload signature (precision, particle count, bunch
+count, map shape), so a serving process tunes once; set PYPAT_AUTOTUNE_CACHE to
+a JSON file path to keep them across processes.  The launch parameters change
+the schedule only: every candidate computes the same results bit for bit.
 
     cfg = autotune_batch(maps, bunches, "fp32_mixed")
     BatchedTracker(maps, "fp32_mixed", **cfg)
@@ -21,23 +23,32 @@ SORT_EVERY = (0, 250, 500, 1000, 2000)
 DEFAULT = {"block_size": 256, "steps_per_launch": 500, "sort_every": 500}
 
 
+_MEMORY = {}
+
+
 def cache_path():
-    return os.environ.get("PYPAT_AUTOTUNE_CACHE",
-                          os.path.join(os.path.expanduser("~"), ".cache", "pypatools_amd", "autotune.json"))
+    """The JSON file of tuned configurations (PYPAT_AUTOTUNE_CACHE), or None: this process only."""
+    return os.environ.get("PYPAT_AUTOTUNE_CACHE") or None
 
 
 def _load_cache():
+    path = cache_path()
+    if path is None:
+        return dict(_MEMORY)
     try:
-        with open(cache_path()) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
 
 
 def _store_cache(cache):
+    _MEMORY.update(cache)
     path = cache_path()
+    if path is None:
+        return
     try:
-        os.makedirs(os.path.dirname(path), exist_ok=True)
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
         with open(path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     except OSError:

@@ -57,9 +57,10 @@ class Bunch:
         return self.r0.shape[0]
 
 
-def make_map(efield, bfield):
-    """The fused 6-component device map of an (efield, bfield) pair on one uniform 3-D grid."""
-    m = _fused_map(efield, bfield)
+def make_map(efield, bfield, precision=None):
+    """The fused 6-component device map of an (efield, bfield) pair on one uniform 3-D grid
+    (for precision 'fp32_mixed' without the FP64 coefficient table)."""
+    m = _fused_map(efield, bfield, fp64=precision != "fp32_mixed")
     if m is None:
         raise ValueError("E and B must be gridded 3-D Fields on the same grid")
     return m
@@ -72,7 +73,7 @@ class BatchedTracker:
             raise ValueError("batched pushes run in 'fp32_mixed' or 'fp64_fast'")
         if not 1 <= len(maps) <= _lib.MAX_BATCH_MAPS:
             raise ValueError(f"1 to {_lib.MAX_BATCH_MAPS} maps")
-        self.maps = [m if isinstance(m, DeviceFieldMap) else make_map(*m) for m in maps]
+        self.maps = [m if isinstance(m, DeviceFieldMap) else make_map(*m, precision=precision) for m in maps]
         self.precision = precision
         self.block_size = int(block_size)
         self.steps_per_launch = int(steps_per_launch)
@@ -158,12 +159,13 @@ class BatchedTracker:
             a.maps[k] = m.handle
         return a
 
-    def _sort(self):
+    def _sort(self, stream=None):
+        """Cell sort within each bunch, enqueued on `stream` (no host synchronisation)."""
         st = self.state
         pa = st._state_args()
         ws = st._workspace()
         _lib.call("pp_sort_state_ex", pa, self.maps[0].handle, self.group.ptr, self.group_shift, self.refine_axis,
-                  self.refine_bits, st.ids.ptr, ws.ptr, ws.nbytes, None)
+                  self.refine_bits, st.ids.ptr, ws.ptr, ws.nbytes, stream)
         st.sorted = True
 
     # ---- running ---------------------------------------------------------------
@@ -184,7 +186,7 @@ class BatchedTracker:
             since = None
             while self.step < end:
                 if self.sort_every > 0 and (since is None or since >= self.sort_every):
-                    self._sort()
+                    self._sort(stream)
                     since = 0
                 k = min(spl, end - self.step)
                 if self.sort_every > 0:
@@ -197,11 +199,14 @@ class BatchedTracker:
         self.timings["run_s"] = self.timings.get("run_s", 0.0) + time.perf_counter() - t0
 
     def _run_graph(self, end):
-        """Capture [launch window, advance counter] once per window length; replay it (sorts stay eager)."""
+        """Capture [launch window, advance counter] once per window length and replay it; the cell sorts
+        between windows are enqueued eagerly on the same stream (the radix sort's library calls stay out
+        of the graph), so the host never waits inside the loop."""
         spl = self.steps_per_launch
         if self.sort_every > 0:
             spl = min(spl, self.sort_every)
         s = Stream()
+        synchronize()                   # work enqueued on the library stream (load, conversions) comes first
         counter = DeviceArray((1,), np.int64)
         counter.upload(np.array([self.step], dtype=np.int64))
         graphs = {}
@@ -222,9 +227,7 @@ class BatchedTracker:
         try:
             while self.step < end:
                 if self.sort_every > 0 and (since is None or since >= self.sort_every):
-                    s.sync()
-                    self._sort()
-                    synchronize()
+                    self._sort(s.handle)        # eager, ordered on the replay stream: no host wait
                     since = 0
                 k = min(spl, end - self.step)
                 if self.sort_every > 0:

@@ -20,7 +20,7 @@ BUILD_DIR = os.path.join(PKG, "_build")
 
 ARCH = os.environ.get("PYPAT_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["pp_host.hip", "pp_strict.hip", "pp_fast.hip", "pp_sort.hip", "pp_moments.hip", "pp_collide.hip",
-           "pp_poisson.hip"]
+           "pp_poisson.hip", "pp_spline.hip"]
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-I", INCLUDE, "-I", CSRC,
@@ -32,6 +32,7 @@ PER_FILE = {
     "pp_strict.hip": ["-ffp-contract=off"],
     "pp_collide.hip": ["-ffp-contract=off"],
     "pp_poisson.hip": ["-ffp-contract=off"],
+    "pp_spline.hip": ["-ffp-contract=off"],
 }
 
 

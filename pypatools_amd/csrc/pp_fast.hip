@@ -167,14 +167,14 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
     } else {
         fe(i, r.x, r.y, r.z, gs, 0, E, B);
         if constexpr (ALG == PP_ALG_BORIS) {
+            // v_new = v + (2 dE + v' x s) with v' = v- + v- x t, v- = v + dE: one FP64 accumulation
             const f3 dE = PRE ? E : h * E;
-            const d3 vm = addf(v, dE);
-            const f3 vmf = tof(vm);
+            const f3 vmf = tof(v) + dE;
             const f3 t = PRE ? B : h * B;
             const f3 s = (2.f * rcpf(1.f + norm2f(t))) * t;
             const f3 vp = vmf + crossf(vmf, t);
-            const f3 dv = crossf(vp, s) + dE;
-            v = addf(vm, dv);
+            const f3 dv = crossf(vp, s) + 2.f * dE;
+            v = addf(v, dv);
             r = r + v * dt;
             vrec = v;
         } else if constexpr (ALG == PP_ALG_BORIS_REL) {
@@ -427,12 +427,14 @@ DEV double hot_axis(double x, double id, double c, int nc, double& t, bool& ok) 
 // (>= 2^29 for nc <= 2^29) into [0, 2^31); q = mulhi(ku, floor((2^32-1)/nc)) is
 // the quotient or one less, so one conditional subtraction leaves k in [0, nc)
 // for every input, and k = floor(u) mod nc whenever |floor(u)| <= 2^29.
+// A power-of-two nc (mask = nc - 1, wave-uniform) wraps with one AND, exact for every clamped floor(u).
 DEV uint32_t hot_axis_periodic(double z, double id, double c, uint32_t nc, uint32_t magic, uint32_t bias,
-                               double& t) {
+                               uint32_t mask, double& t) {
     const double u = fma(z, id, c);
     const double fu = floor(u);
     t = u - fu;
     const int32_t ki = (int32_t)fmin(fmax(fu, -536870912.0), 536870912.0);
+    if (mask != 0u) return (uint32_t)ki & mask;
     const uint32_t ku = (uint32_t)ki + bias;
     uint32_t k = ku - __umulhi(ku, magic) * nc;
     if (k >= nc) k -= nc;
@@ -450,12 +452,99 @@ DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& t
     const double i = hot_axis(x, m.id0, m.c0, m.nc0, tx, ok);
     const double j = hot_axis(y, m.id1, m.c1, m.nc1, ty, ok);
     double k;
-    if constexpr (PERIODIC) k = (double)hot_axis_periodic(z, m.id2, m.c2, (uint32_t)m.nc2, m.zmagic, m.zbias, tz);
+    if constexpr (PERIODIC)
+        k = (double)hot_axis_periodic(z, m.id2, m.c2, (uint32_t)m.nc2, m.zmagic, m.zbias, m.zmask, tz);
     else k = hot_axis(z, m.id2, m.c2, m.nc2, tz, ok);
     return (uint32_t)fma(i, m.nc12d, fma(j, m.nc2d, k));
 }
 
+// LDS staging of a wavefront's cells: up to kStageSlots distinct cells per gather,
+// each cell as PIECES x 16 B (12 for the FP32 table, 24 for FP64), padded to
+// PIECES + 1 so that slots start in different LDS banks.
+// 16 slots cover the distinct cells of nearly every wavefront of a cell-sorted C2 bunch up to the next
+// sort 500 steps later (a CPU replay of the bunch: 2.2 distinct cells per wavefront one step after the
+// sort, 7.4 after 250 steps, 9.5 after 500 steps with 35 % of wavefronts above 10 and the 90th
+// percentile at 13); a wavefront over more cells gathers per lane.
+constexpr int kStageSlots = 16;
+constexpr int kStageWaves = 4;      // wavefronts per block (the hot kernels launch <= 256 threads)
+
+// Staging roles of the lanes of a full wavefront, fixed for the whole launch.  In each
+// pass, lane l loads piece l % PIECES of the cell in slot l / PIECES (CPP = 64 / PIECES
+// cells per pass; lanes beyond CPP * PIECES stage nothing).
+template <int PIECES>
+struct StageLanes {
+    static constexpr int kCPP = 64 / PIECES;
+    static constexpr int kPitch = PIECES + 1;   // 16 B units per slot
+    uint32_t slot;   // l / PIECES, 64 for lanes that stage nothing
+    uint32_t src;    // byte offset of the lane's piece within its cell
+    uint32_t dst;    // 16 B unit of the lane's piece in the wavefront's LDS area, pass 0
+};
+
+template <int PIECES>
+DEV StageLanes<PIECES> stage_lanes() {
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const uint32_t sl = lane / PIECES, part = lane - sl * PIECES;
+    StageLanes<PIECES> s;
+    s.slot = sl < (uint32_t)StageLanes<PIECES>::kCPP ? sl : 64u;
+    s.src = part * 16u;
+    s.dst = sl * StageLanes<PIECES>::kPitch + part;
+    return s;
+}
+
 #if defined(__HIP_DEVICE_COMPILE__)
+// The distinct cells of a full wavefront, at most kStageSlots of them: each lane's slot in
+// `myslot`.  Returns their number, or 0 when the wavefront spans more cells than that.  The scan
+// is unrolled at compile time (template recursion): per distinct cell one readlane, one compare,
+// one select, and scalar mask updates with no loop counter (measured: the rolled loop ran C2
+// at 1.57e11 particle-steps/s, this form at 1.89e11).
+template <int IT>
+DEV void distinct_step(uint32_t cell, uint64_t& todo, uint32_t& myslot, int& ns) {
+    if constexpr (IT < kStageSlots) {
+        if (todo == 0) return;
+        const uint32_t c = __builtin_amdgcn_readlane(cell, (int)__builtin_ctzll(todo));
+        const bool other = cell != c;
+        todo &= __builtin_amdgcn_ballot_w64(other);
+        myslot = other ? myslot : (uint32_t)IT;
+        ns = IT + 1;
+        distinct_step<IT + 1>(cell, todo, myslot, ns);
+    }
+}
+
+DEV int distinct_cells(uint32_t cell, uint32_t& myslot) {
+    uint64_t todo = ~0ull;
+    int ns = 0;
+    distinct_step<0>(cell, todo, myslot, ns);
+    return todo == 0 ? ns : 0;
+}
+
+// Loads the wavefront's ns distinct cells (cell_bytes each) into its LDS slots: every lane
+// writes its cell to its slot's entry of the wavefront's cell list (lanes of one slot write the
+// same value), then each staging lane reads the cell of its slot from the list and loads its
+// piece.  No block barrier is needed: the slots and the list are private to the wavefront and
+// its LDS operations execute in order (the wavefront fence and wave_barrier keep the compiler
+// from reordering them).
+template <int PIECES>
+DEV void stage_cells(float4* wv, uint32_t* list, __amdgpu_buffer_rsrc_t rs, uint32_t cell_bytes, int ns,
+                     uint32_t cell, uint32_t myslot, const StageLanes<PIECES>& L) {
+    constexpr int kCPP = StageLanes<PIECES>::kCPP;
+    constexpr int kPitch = StageLanes<PIECES>::kPitch;
+    list[myslot] = cell;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int p = 0; p * kCPP < kStageSlots; ++p) {
+        if (p * kCPP >= ns) break;
+        if (L.slot + (uint32_t)(kCPP * p) < (uint32_t)ns) {
+            const uint32_t c = list[L.slot + kCPP * p];
+            wv[L.dst + kCPP * kPitch * p] = ld4f(rs, __umul24(c, cell_bytes) + L.src);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Constant-address-space view of the coefficient table: loads at a wave-uniform
 // address become scalar loads (s_load_dwordx16) into SGPRs.
 typedef const __attribute__((address_space(4))) float* cfloat_p;
@@ -471,27 +560,32 @@ DEV float ceval_s(cfloat_p q, const Mono& w) {
 }
 #endif
 
-// Field multipliers of the FP32 hot gather: E = kE * e and B = kB * b inside the
-// map, kE * fill and kB * fill outside.  kE, kB are the field scales, times
-// h = q/m dt / 2 for the Boris-family steps (step_mixed<ALG, true>).
+DEV double2 as_d2(float4 v) {
+    double2 d;
+    d.x = __hiloint2double((int)__float_as_uint(v.y), (int)__float_as_uint(v.x));
+    d.y = __hiloint2double((int)__float_as_uint(v.w), (int)__float_as_uint(v.z));
+    return d;
+}
+
+// Field multipliers of the FP32 hot gather: E = kE * e and B = kB * b.  kE, kB are the
+// field scales, times h = q/m dt / 2 for the Boris-family steps (step_mixed<ALG, true>).
 struct HotScale {
-    float kE, kB, kEf, kBf;
+    float kE, kB;
 };
 
 DEV HotScale hot_scale(const HotMap& m, double h) {
     HotScale s;
     s.kE = (float)(m.sE * h);
     s.kB = (float)(m.sB * h);
-    s.kEf = s.kE * (float)m.fill;
-    s.kBf = s.kB * (float)m.fill;
     return s;
 }
 
-// LDS staging of a wavefront's cells: up to kStageSlots distinct cells, 12 x 16 B
-// each, padded to 13 x 16 B so that slots fall in different LDS banks.
-constexpr int kStageSlots = 10;
-constexpr int kStageWaves = 4;      // wavefronts per block (the hot kernels launch <= 256 threads)
-constexpr int kStagePitch = 13;     // float4 per slot
+// Points outside the map gather from a cell index past the end of the table: its buffer loads
+// return 0, which is the field there (the hot kernels take only maps that fill 0 outside).  The
+// byte offsets of these indices (24-bit products) exceed every table the hot paths accept
+// (below 2^31 bytes: fewer than 0xFFFFFF cells of 192 B, 0x7FFFFF of 384 B).
+constexpr uint32_t kOutside32 = 0xFFFFFFu;
+constexpr uint32_t kOutside64 = 0x7FFFFFu;
 
 // RF factor of one field at global step gs and RK stage (wave-uniform: a scalar load)
 DEV double hot_rf(const HotRF& rf, int mode, int64_t gs, int stage) {
@@ -507,21 +601,26 @@ struct HotFE32 {
     __amdgpu_buffer_rsrc_t rs;
     HotScale k;
     HotRF rf;
+    StageLanes<12> lanes = stage_lanes<12>();
     mutable uint32_t last_cell = 0;   // REUSE: this lane's cell and LDS slot at the last staging
     mutable uint32_t last_slot = 0;
     mutable bool staged = false;      // REUSE: the wavefront has staged at least once (wave-uniform)
     DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B) const {
         double tx, ty, tz;
         bool ok;
-        const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
+        uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
+        cell = ok ? cell : kOutside32;
         const Mono w = monomials((float)tx, (float)ty, (float)tz);
         float v[6];
         bool done = false;
 #if defined(__HIP_DEVICE_COMPILE__)
-        __shared__ float4 stage_lds[kStageWaves][kStageSlots * kStagePitch];
+        constexpr int kPitch = StageLanes<12>::kPitch;
+        __shared__ float4 stage_lds[kStageWaves][kStageSlots * kPitch];
+        __shared__ uint32_t stage_list[kStageWaves][kStageSlots];
+        float4* wv = stage_lds[threadIdx.x >> 6];
         if constexpr (REUSE) {
             if (staged && __builtin_amdgcn_ballot_w64(cell != last_cell) == 0) {
-                const float4* q = stage_lds[threadIdx.x >> 6] + last_slot * kStagePitch;
+                const float4* q = wv + __umul24(last_slot, (uint32_t)kPitch);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
                 done = true;
@@ -537,38 +636,17 @@ struct HotFE32 {
         // All three evaluate the same FMA chain on the same values.
         const uint32_t c0 = __builtin_amdgcn_readfirstlane(cell);
         if (done) {
-        } else if (__builtin_amdgcn_ballot_w64(cell != c0) == 0) {
+        } else if (__builtin_amdgcn_ballot_w64(cell != c0) == 0 && c0 < m.ncells) {
             const cfloat_p q = (cfloat_p)m.coef + (size_t)c0 * 48u;
 #pragma unroll
             for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, w);
             done = true;
         } else if (__builtin_amdgcn_ballot_w64(true) == ~0ull) {
-            const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-            const uint32_t lslot = lane / 12u, lpart = lane - 12u * lslot;   // staging role of this lane
-            uint64_t todo = ~0ull;
-            uint32_t myslot = 0, ld0 = 0, ld1 = 0;
-            int ns = 0;
-#pragma unroll
-            for (int it = 0; it < kStageSlots; ++it) {
-                if (todo == 0) break;
-                const uint32_t c = __builtin_amdgcn_readlane(cell, (int)__builtin_ctzll(todo));
-                const bool mine = cell == c;
-                todo &= ~__builtin_amdgcn_ballot_w64(mine);
-                myslot = mine ? (uint32_t)it : myslot;
-                if (it < 5) ld0 = lslot == (uint32_t)it ? c : ld0;
-                else ld1 = lslot == (uint32_t)(it - 5) ? c : ld1;
-                ns = it + 1;
-            }
-            if (todo == 0) {
-                float4* wv = stage_lds[threadIdx.x >> 6];
-                if (lslot < 5u && lslot < (uint32_t)ns)
-                    wv[lslot * kStagePitch + lpart] = ld4f(rs, __umul24(ld0, 192u) + lpart * 16u);
-                if (lslot < 5u && lslot + 5u < (uint32_t)ns)
-                    wv[(lslot + 5u) * kStagePitch + lpart] = ld4f(rs, __umul24(ld1, 192u) + lpart * 16u);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const float4* q = wv + myslot * kStagePitch;
+            uint32_t myslot = 0;
+            const int ns = distinct_cells(cell, myslot);
+            if (ns > 0) {
+                stage_cells<12>(wv, stage_list[threadIdx.x >> 6], rs, 192u, ns, cell, myslot, lanes);
+                const float4* q = wv + __umul24(myslot, (uint32_t)kPitch);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
                 __builtin_amdgcn_wave_barrier();
@@ -587,50 +665,83 @@ struct HotFE32 {
             for (int c = 0; c < 6; ++c)
                 v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
         }
-        float kE = k.kE, kEf = k.kEf, kB = k.kB, kBf = k.kBf;
+        float kE = k.kE, kB = k.kB;
         if constexpr (RF) {
-            if (rf.e != PP_RF_NONE) {
-                const float f = (float)hot_rf(rf, rf.e, gs, stage);
-                kE *= f;
-                kEf *= f;
-            }
-            if (rf.b != PP_RF_NONE) {
-                const float f = (float)hot_rf(rf, rf.b, gs, stage);
-                kB *= f;
-                kBf *= f;
-            }
+            if (rf.e != PP_RF_NONE) kE *= (float)hot_rf(rf, rf.e, gs, stage);
+            if (rf.b != PP_RF_NONE) kB *= (float)hot_rf(rf, rf.b, gs, stage);
         }
-        const float me = ok ? kE : 0.f, ae = ok ? 0.f : kEf;
-        const float mb = ok ? kB : 0.f, ab = ok ? 0.f : kBf;
-        E = mkf(fmaf(v[0], me, ae), fmaf(v[1], me, ae), fmaf(v[2], me, ae));
-        B = mkf(fmaf(v[3], mb, ab), fmaf(v[4], mb, ab), fmaf(v[5], mb, ab));
+        E = mkf(kE * v[0], kE * v[1], kE * v[2]);
+        B = mkf(kB * v[3], kB * v[4], kB * v[5]);
     }
 };
 
-template <bool PERIODIC, bool RF = false>
+// FP64-fast gather: 384 B cells ([comp][8 doubles]).  A full wavefront over at most
+// kStageSlots distinct cells stages them through LDS (24 lanes x 16 B per cell, two cells
+// per load instruction); other wavefronts gather per lane.  Both read the same doubles.
+template <bool PERIODIC, bool RF = false, bool REUSE = false>
 struct HotFE64 {
     const HotMap& m;
     __amdgpu_buffer_rsrc_t rs;
     HotRF rf;
+    StageLanes<24> lanes = stage_lanes<24>();
+    mutable uint32_t last_cell = 0;
+    mutable uint32_t last_slot = 0;
+    mutable bool staged = false;
     DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, d3& E, d3& B) const {
         double tx, ty, tz;
         bool ok;
-        const uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
-        const uint32_t base = __umul24(cell, 48u * 8u);
+        uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
+        cell = ok ? cell : kOutside64;
         double v[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const uint32_t o = base + c * 64u;
-            v[c] = cevald(ld2d(rs, o), ld2d(rs, o + 16u), ld2d(rs, o + 32u), ld2d(rs, o + 48u), tx, ty, tz);
+        bool done = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+        constexpr int kPitch = StageLanes<24>::kPitch;
+        __shared__ float4 stage_lds64[kStageWaves][kStageSlots * kPitch];
+        __shared__ uint32_t stage_list64[kStageWaves][kStageSlots];
+        float4* wv = stage_lds64[threadIdx.x >> 6];
+        uint32_t slot = 0;
+        if constexpr (REUSE) {
+            if (staged && __builtin_amdgcn_ballot_w64(cell != last_cell) == 0) {
+                slot = last_slot;
+                done = true;
+            }
         }
-        const double fl = m.fill;
+        if (!done && __builtin_amdgcn_ballot_w64(true) == ~0ull) {
+            const int ns = distinct_cells(cell, slot);
+            if (ns > 0) {
+                stage_cells<24>(wv, stage_list64[threadIdx.x >> 6], rs, 384u, ns, cell, slot, lanes);
+                done = true;
+                if constexpr (REUSE) {
+                    last_cell = cell;
+                    last_slot = slot;
+                    staged = true;
+                }
+            }
+        }
+        if (done) {
+            const float4* q = wv + __umul24(slot, (uint32_t)kPitch);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                v[c] = cevald(as_d2(q[4 * c]), as_d2(q[4 * c + 1]), as_d2(q[4 * c + 2]), as_d2(q[4 * c + 3]), tx, ty,
+                              tz);
+            __builtin_amdgcn_wave_barrier();
+        }
+#endif
+        if (!done) {
+            const uint32_t base = __umul24(cell, 48u * 8u);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const uint32_t o = base + c * 64u;
+                v[c] = cevald(ld2d(rs, o), ld2d(rs, o + 16u), ld2d(rs, o + 32u), ld2d(rs, o + 48u), tx, ty, tz);
+            }
+        }
         double sE = m.sE, sB = m.sB;
         if constexpr (RF) {
             if (rf.e != PP_RF_NONE) sE *= hot_rf(rf, rf.e, gs, stage);
             if (rf.b != PP_RF_NONE) sB *= hot_rf(rf, rf.b, gs, stage);
         }
-        E = mk3(sE * (ok ? v[0] : fl), sE * (ok ? v[1] : fl), sE * (ok ? v[2] : fl));
-        B = mk3(sB * (ok ? v[3] : fl), sB * (ok ? v[4] : fl), sB * (ok ? v[5] : fl));
+        E = mk3(sE * v[0], sE * v[1], sE * v[2]);
+        B = mk3(sB * v[3], sB * v[4], sB * v[5]);
     }
 };
 
@@ -681,13 +792,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     constexpr bool PRE = kPrescaled<ALG>;
     constexpr bool REUSE = ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL;
     const HotFE32<PERIODIC, RF, REUSE> fe32{p.m, rs, hot_scale(p.m, PRE ? 0.5 * p.qm * dt : 1.0), p.rf};
+    const HotFE64<PERIODIC, RF, REUSE> fe64{p.m, rs, p.rf};
     const bool rec_on = p.r_hist != nullptr;
     int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
-        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, HotFE64<PERIODIC, RF>{p.m, rs, p.rf}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, fe64, ac);
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
             write_record_to(p.r_hist, p.v_hist, p.n, p.n_records, p.ids, rec_idx, i, r, vrec);
@@ -728,10 +840,12 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
     d3 vrec = v;
     const float h = (float)(0.5 * ac.qm * dt);
     constexpr bool PRE = kPrescaled<ALG>;
-    const HotFE32<PERIODIC, false, ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0), HotRF{}};
+    constexpr bool REUSE = ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL;
+    const HotFE32<PERIODIC, false, REUSE> fe32{m, rs, hot_scale(m, PRE ? 0.5 * ac.qm * dt : 1.0), HotRF{}};
+    const HotFE64<PERIODIC, false, REUSE> fe64{m, rs, HotRF{}};
     for (int64_t s = 0; s < ns; ++s) {
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, step0 + s, fe32, ac);
-        else step64<ALG>(i, r, v, vrec, dt, step0 + s, HotFE64<PERIODIC>{m, rs, HotRF{}}, ac);
+        else step64<ALG>(i, r, v, vrec, dt, step0 + s, fe64, ac);
     }
     if constexpr (STAGGERED) {
         if (kick_end) v = batch_kick<ALG, MIXED, PERIODIC>(i, r, v, 0.5 * dt, m, rs, ac);

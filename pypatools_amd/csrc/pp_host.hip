@@ -202,6 +202,55 @@ int pp_event_record(pp_event e, pp_stream s) {
     return PP_OK;
 }
 
+int pp_host_alloc(void** hptr, size_t bytes) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!hptr) return fail(PP_ERR_INVALID, "null output pointer");
+    *hptr = nullptr;
+    HIP_TRY(hipHostMalloc(hptr, bytes ? bytes : 256, hipHostMallocDefault), "hipHostMalloc");
+    return PP_OK;
+}
+
+int pp_host_free(void* hptr) {
+    if (!hptr) return PP_OK;
+    HIP_TRY(hipHostFree(hptr), "hipHostFree");
+    return PP_OK;
+}
+
+int pp_memcpy_h2d_async(void* dst, const void* src, size_t bytes, pp_stream s) {
+    if (!bytes) return PP_OK;
+    if (!dst || !src) return fail(PP_ERR_INVALID, "null pointer");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_of(s)), "hipMemcpyAsync H2D");
+    return PP_OK;
+}
+
+int pp_memcpy_d2h_async(void* dst, const void* src, size_t bytes, pp_stream s) {
+    if (!bytes) return PP_OK;
+    if (!dst || !src) return fail(PP_ERR_INVALID, "null pointer");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_of(s)), "hipMemcpyAsync D2H");
+    return PP_OK;
+}
+
+int pp_event_query(pp_event e, int32_t* done) {
+    if (!e || !done) return fail(PP_ERR_INVALID, "null argument");
+    const hipError_t r = hipEventQuery(static_cast<hipEvent_t>(e));
+    if (r == hipSuccess) {
+        *done = 1;
+        return PP_OK;
+    }
+    if (r == hipErrorNotReady) {
+        *done = 0;
+        return PP_OK;
+    }
+    return hip_fail(r, "hipEventQuery");
+}
+
+int pp_event_sync(pp_event e) {
+    if (!e) return fail(PP_ERR_INVALID, "null event");
+    HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(e)), "hipEventSynchronize");
+    return PP_OK;
+}
+
 int pp_event_elapsed_ms(float* ms, pp_event start, pp_event stop) {
     HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(stop)), "hipEventSynchronize");
     HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)),
@@ -315,9 +364,13 @@ int pp_map_create(const double* values, int32_t ncomp, int32_t nx, int32_t ny, i
     const int64_t ncell = (int64_t)d.nc[0] * d.nc[1] * d.nc[2];
     double* c64 = nullptr;
     float* c32 = nullptr;
+    if (!(flags & PP_MAP_NO_COEF) && (flags & PP_MAP_NO_FP64) && (flags & PP_MAP_NO_FP32))
+        return fail(PP_ERR_INVALID, "PP_MAP_NO_FP64 with PP_MAP_NO_FP32 leaves no coefficient table: use PP_MAP_NO_COEF");
     if (!(flags & PP_MAP_NO_COEF)) {
-        if ((rc = map_alloc(m, sizeof(double) * ncell * ncomp * 8, &p))) { pp_map_destroy(m); return rc; }
-        c64 = static_cast<double*>(p);
+        if (!(flags & PP_MAP_NO_FP64)) {
+            if ((rc = map_alloc(m, sizeof(double) * ncell * ncomp * 8, &p))) { pp_map_destroy(m); return rc; }
+            c64 = static_cast<double*>(p);
+        }
         if (!(flags & PP_MAP_NO_FP32)) {
             if ((rc = map_alloc(m, sizeof(float) * ncell * ncomp * 8, &p))) { pp_map_destroy(m); return rc; }
             c32 = static_cast<float*>(p);
@@ -372,7 +425,7 @@ static int make_field(const pp_field& f, int precision, FieldDev& out) {
                 return fail(PP_ERR_UNSUPPORTED, "precision %d needs a uniform grid; use PP_PREC_FP64 for "
                             "non-uniform grids", precision);
             if (precision == PP_PREC_FP64_FAST && !out.map.coef64)
-                return fail(PP_ERR_UNSUPPORTED, "map was built without coefficient tables");
+                return fail(PP_ERR_UNSUPPORTED, "map was built without FP64 coefficient tables");
             if (precision == PP_PREC_FP32_MIXED && !out.map.coef32)
                 return fail(PP_ERR_UNSUPPORTED, "map was built without FP32 tables");
         }
@@ -398,10 +451,12 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
     if (m.nc[2] > 0) {
         h.zmagic = (uint32_t)(0xFFFFFFFFull / (uint64_t)m.nc[2]);
         h.zbias = (uint32_t)m.nc[2] * ((1u << 30) / (uint32_t)m.nc[2]);
+        if ((m.nc[2] & (m.nc[2] - 1)) == 0) h.zmask = (uint32_t)m.nc[2] - 1u;
     }
     h.sE = 1.0;
     h.sB = 1.0;
     h.fill = m.fill;
+    h.ncells = (uint32_t)((uint64_t)m.nc[0] * m.nc[1] * m.nc[2]);
 }
 
 // The hot kernels cover the common case: E and B from one uniform 3-D map built
@@ -418,6 +473,7 @@ static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& 
     if ((E.rf || B.rf) && !p.rf_table) return false;
     if (E.has_offset || B.has_offset || E.comp0 != 0 || B.comp0 != 3 || m.ncomp != 6) return false;
     if (!m.uniform || m.n[0] < 2 || m.n[1] < 2 || m.n[2] < 2) return false;
+    if (m.fill != 0.0) return false;        // the hot gathers read 0 outside the map
     const bool mixed = a->precision == PP_PREC_FP32_MIXED;
     const void* coef = mixed ? (const void*)m.coef32 : (const void*)m.coef64;
     if (!coef) return false;
@@ -450,6 +506,58 @@ static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& 
     hp.m.sE = E.has_scale2 ? E.scale1 * E.scale2 : E.scale1;
     hp.m.sB = B.has_scale2 ? B.scale1 * B.scale2 : B.scale1;
     return true;
+}
+
+int pp_spline_create(const double* values, int32_t ncomp, int32_t ndim, const int64_t* shape, const double* axis0,
+                     const double* axis1, const double* axis2, int32_t order, int32_t flags, double cval,
+                     pp_spline** out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!values || !shape || !out) return fail(PP_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (ncomp < 1) return fail(PP_ERR_INVALID, "ncomp must be positive");
+    if (ndim < 1 || ndim > 3) return fail(PP_ERR_INVALID, "Only 1D, 2D, 3D supported. Got %dD", ndim);
+    if (order != 0 && order != 1 && order != 3 && order != 5)
+        return fail(PP_ERR_INVALID, "spline order must be 0, 1, 3 or 5 (got %d)", order);
+    if (flags & ~(PP_SPLINE_NEAREST_MODE | PP_SPLINE_RGI)) return fail(PP_ERR_INVALID, "unknown spline flags");
+    const bool rgi = (flags & PP_SPLINE_RGI) != 0;
+    if (rgi && order != 0) return fail(PP_ERR_INVALID, "PP_SPLINE_RGI is the order-0 rule of RegularGridInterpolator");
+    const double* axes[3] = {axis0, axis1, axis2};
+    int64_t total = ncomp;
+    for (int a = 0; a < ndim; ++a) {
+        if (!axes[a]) return fail(PP_ERR_INVALID, "axis %d is null", a);
+        if (shape[a] < 1 || shape[a] > ((int64_t)1 << 31)) return fail(PP_ERR_INVALID, "axis %d: bad node count", a);
+        if (rgi && shape[a] < 2) return fail(PP_ERR_INVALID, "axis %d: RegularGridInterpolator needs 2 nodes", a);
+        for (int64_t k = 1; k < shape[a]; ++k)
+            if (!(axes[a][k] > axes[a][k - 1]))
+                return fail(PP_ERR_INVALID, "axis %d: node coordinates must be strictly increasing", a);
+        total *= shape[a];
+        if (total > ((int64_t)1 << 33)) return fail(PP_ERR_INVALID, "grid too large");
+    }
+    hipError_t e = pp_spl::create(values, ncomp, ndim, shape, axes, order, (flags & PP_SPLINE_NEAREST_MODE) != 0, rgi,
+                                  cval, g_device, out);
+    if (e != hipSuccess) return hip_fail(e, "spline creation");
+    return PP_OK;
+}
+
+int pp_spline_destroy(pp_spline* s) {
+    pp_spl::destroy(s);
+    return PP_OK;
+}
+
+int pp_spline_eval(const pp_spline* s, const double* pts, int64_t m, int32_t stride, int32_t col0, double* out,
+                   pp_stream st) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!s) return fail(PP_ERR_INVALID, "null spline");
+    if (m < 0) return fail(PP_ERR_INVALID, "negative point count");
+    if (m == 0) return PP_OK;
+    if (!pts || !out) return fail(PP_ERR_INVALID, "null device pointer");
+    if (col0 < 0 || stride < 1 || col0 + s->d.ndim > stride)
+        return fail(PP_ERR_INVALID, "columns %d..%d do not fit a row of %d", col0, col0 + s->d.ndim - 1, stride);
+    if (s->device != g_device) return fail(PP_ERR_INVALID, "spline lives on another device");
+    HIP_TRY(pp_spl::eval(s, pts, m, stride, col0, out, stream_of(st)), "spline eval");
+    return PP_OK;
 }
 
 int pp_field_eval(const pp_field* field, int32_t precision, const double* pts, int64_t m, double rf_value,
@@ -607,6 +715,7 @@ int pp_push_batched(const pp_batch_args* a, pp_stream s) {
         if (mp->device != g_device) return fail(PP_ERR_INVALID, "map %d lives on another device", k);
         if (m.ncomp != 6 || !m.uniform || m.n[0] < 2 || m.n[1] < 2 || m.n[2] < 2)
             return fail(PP_ERR_UNSUPPORTED, "map %d: batched pushes need 6-component uniform 3-D maps", k);
+        if (m.fill != 0.0) return fail(PP_ERR_UNSUPPORTED, "map %d: batched pushes need maps that fill 0 outside", k);
         const void* coef = mixed ? (const void*)m.coef32 : (const void*)m.coef64;
         if (!coef) return fail(PP_ERR_UNSUPPORTED, "map %d has no coefficient table for this precision", k);
         const uint64_t bytes = (uint64_t)m.nc[0] * m.nc[1] * m.nc[2] * 48 * (mixed ? 4 : 8);

@@ -76,9 +76,11 @@ struct HotMap {
     double id0, id1, id2;   // inv_d[a]
     double nc2d, nc12d;     // nc2 and nc1 * nc2 as doubles: cell = i nc12d + j nc2d + k, exact in FP64
     double sE, sB;          // scale1 * scale2 of E and B
-    double fill;            // value outside the grid (Field: 0)
+    double fill;            // value outside the grid: 0 (the hot kernels take only such maps)
+    uint32_t ncells;        // cells in the table (nc0 nc1 nc2)
     uint32_t zmagic;        // floor((2^32 - 1) / nc2): quotient estimate for the periodic z wrap
     uint32_t zbias;         // nc2 * floor(2^30 / nc2): makes the z cell index non-negative before the wrap
+    uint32_t zmask;         // nc2 - 1 when nc2 is a power of two (the wrap is then one AND), else 0
 };
 
 // RF factors of the hot kernels: table [step - step0][stage][cos, sin]; e, b = pp_rf_mode of E and B
@@ -171,6 +173,34 @@ struct MomentArgs {
 };
 
 constexpr int kMomentBlocks = 1024;   // partials per reduction pass
+
+// spline interpolators (pp_spline.hip)
+namespace pp_splk {
+struct SplineDesc {
+    const double* coef;        // [ncomp][N0][N1][N2], last axis fastest (N = n + 2 pad)
+    const double* axis[3];     // node coordinates, n[a] each
+    int64_t n[3];              // nodes per axis (1 for unused axes)
+    int64_t N[3];              // coefficient extent per axis
+    int64_t ncomp;
+    int32_t ndim, order, nearest, rgi, pad;
+    double cval;
+};
+}  // namespace pp_splk
+
+struct pp_spline {
+    pp_splk::SplineDesc d;
+    void* allocs[4];
+    int nalloc;
+    int device;
+};
+
+namespace pp_spl {
+void destroy(pp_spline* s);
+hipError_t create(const double* values, int64_t ncomp, int ndim, const int64_t* n, const double* const* axes, int order,
+                  int nearest, int rgi, double cval, int device, pp_spline** out);
+hipError_t eval(const pp_spline* s, const double* pts, int64_t m, int32_t stride, int32_t col0, double* out,
+                hipStream_t st);
+}  // namespace pp_spl
 
 namespace pp_beam {
 size_t workspace_bytes();

@@ -95,6 +95,44 @@ class DeviceArray:
         self.free()
 
 
+class PinnedArray:
+    """Page-locked host memory (pp_host_alloc) with a NumPy view, for asynchronous copies."""
+
+    def __init__(self, nbytes):
+        self.nbytes = max(int(nbytes), 1)
+        p = ctypes.c_void_p()
+        _lib.call("pp_host_alloc", ctypes.byref(p), self.nbytes)
+        self.ptr = p.value
+        self._buf = (ctypes.c_uint8 * self.nbytes).from_address(self.ptr)
+
+    def view(self, dtype, count, offset_bytes=0):
+        """A NumPy array of `count` items of `dtype` over this memory from `offset_bytes`."""
+        dtype = np.dtype(dtype)
+        if offset_bytes + count * dtype.itemsize > self.nbytes:
+            raise ValueError("view beyond the pinned buffer")
+        return np.frombuffer(self._buf, dtype=dtype, count=count, offset=offset_bytes)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self._buf = None
+            try:
+                _lib.lib().pp_host_free(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def copy_h2d_async(dst_ptr, pinned, nbytes, stream=None, offset_bytes=0):
+    _lib.call("pp_memcpy_h2d_async", dst_ptr, pinned.ptr + offset_bytes, nbytes, stream)
+
+
+def copy_d2h_async(pinned, src_ptr, nbytes, stream=None, offset_bytes=0):
+    _lib.call("pp_memcpy_d2h_async", pinned.ptr + offset_bytes, src_ptr, nbytes, stream)
+
+
 class Stream:
     def __init__(self):
         h = ctypes.c_void_p()
@@ -126,6 +164,15 @@ class Event:
         ms = ctypes.c_float()
         _lib.call("pp_event_elapsed_ms", ctypes.byref(ms), self.handle, stop.handle)
         return float(ms.value)
+
+    def query(self):
+        """True once the work recorded before this event has completed (does not block)."""
+        done = ctypes.c_int32()
+        _lib.call("pp_event_query", self.handle, ctypes.byref(done))
+        return bool(done.value)
+
+    def sync(self):
+        _lib.call("pp_event_sync", self.handle)
 
     def __del__(self):
         if getattr(self, "handle", None):

@@ -35,7 +35,10 @@ DEFAULT_LAUNCH_WORK = 2 ** 30
 DEFAULT_SORT_EVERY = 500
 SORT_MIN_PARTICLES = 4096
 # sub-cell bins (2^bits) along the direction of motion in the sort key
-DEFAULT_REFINE_BITS = 0
+# cell sorts also order each cell's particles in 2^3 bins along the direction of motion: a wavefront
+# then starts in fewer cells and keeps to fewer (C2: 1.3 distinct cells per wavefront one step after a
+# sort instead of 2.2, 4.0 after 100 steps instead of 5.1; 1.93e11 particle-steps/s against 1.87e11)
+DEFAULT_REFINE_BITS = 3
 
 
 def motion_axis(v):
@@ -174,8 +177,9 @@ def _base_field(f):
 _FUSED_CACHE = {}
 
 
-def _fused_map(fe, fb):
-    """One 6-component device map for E and B on the same grid (one cell lookup per gather)."""
+def _fused_map(fe, fb, fp64=True):
+    """One 6-component device map for E and B on the same grid (one cell lookup per gather).
+    fp64=False: without the FP64 coefficient table (maps used by FP32-mixed launches only)."""
     be, bb = _base_field(fe), _base_field(fb)
     if be is None or bb is None or be is bb:
         return None
@@ -184,11 +188,11 @@ def _fused_map(fe, fb):
         return None
     if not all(np.array_equal(a, b) for a, b in zip(se[0], sb[0])):
         return None
-    key = (id(be), id(bb))
+    key = (id(be), id(bb), bool(fp64))
     hit = _FUSED_CACHE.get(key)
     if hit is not None and hit[0] is be and hit[1] is bb:
         return hit[2]
-    m = DeviceFieldMap(se[0], se[1] + sb[1], 0.0, se[2])
+    m = DeviceFieldMap(se[0], se[1] + sb[1], 0.0, se[2], fp64=fp64)
     _FUSED_CACHE[key] = (be, bb, m)
     return m
 

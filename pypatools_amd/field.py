@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray
-from .interpolators import BACKENDS, DeviceFieldMap, HipInterpolator, eval_term, get_interpolator
+from .interpolators import BACKENDS, DeviceFieldMap, DeviceSpline, HipInterpolator, eval_term, get_interpolator
 
 UNIT_SCALES = {
     "m": 1.0, "cm": 0.01, "mm": 0.001,
@@ -93,6 +93,7 @@ class Field(FieldBase):
         if field is not None:
             self._field = field
         self._dev_map = None
+        self._dev_spline = None
 
     # ---- core API ----------------------------------------------------------
     def __call__(self, pts):
@@ -226,6 +227,15 @@ class Field(FieldBase):
         """(points, [vx, vy, vz], periodic) of a gridded field, or None."""
         if self._is_zero or self._dim == 0:
             return None
+        comps = self._shared_grid_components()
+        if comps is None or any(c.order != 1 for c in comps):
+            return None
+        return comps[0].grid, [c.values for c in comps], self._periodic_z
+
+    def _shared_grid_components(self):
+        """The three HipInterpolator components when they share one grid, else None."""
+        if self._is_zero or self._dim == 0:
+            return None
         comps = [self._field.get(c) for c in "xyz"]
         if not all(isinstance(c, HipInterpolator) for c in comps):
             return None
@@ -233,7 +243,22 @@ class Field(FieldBase):
         for c in comps[1:]:
             if len(c.grid) != len(g0) or not all(np.array_equal(a, b) for a, b in zip(c.grid, g0)):
                 return None
-        return g0, [c.values for c in comps], self._periodic_z
+        return comps
+
+    def device_spline(self):
+        """One 3-component device spline for a field whose components share a grid and a spline
+        order other than 1 (method 'nearest', 'cubic', 'quintic'), else None."""
+        if self._dev_spline is None:
+            comps = self._shared_grid_components()
+            if comps is None or comps[0].order == 1 or any(c.order != comps[0].order or c.rgi != comps[0].rgi
+                                                            or c.fill_value != comps[0].fill_value for c in comps):
+                return None
+            if self._periodic_z:
+                raise NotImplementedError("periodic_z maps are interpolated linearly only")
+            c0 = comps[0]
+            self._dev_spline = DeviceSpline(c0.grid, [c.values for c in comps], c0.order, rgi=c0.rgi,
+                                            cval=c0.fill_value)
+        return self._dev_spline
 
     def device_map(self):
         if self._dev_map is None:
@@ -433,6 +458,11 @@ def evaluate_on_device(f, pts_dev, m, precision, t=None):
         for g, w in zip(f.fields, f.weights):
             val = evaluate_on_device(g, pts_dev, m, precision, t)
             _lib.call("pp_axpy_batch", out.ptr, float(w), None, val.ptr, m, out.ptr, None)
+        return out
+    if isinstance(f, Field) and f.device_spline() is not None:
+        # spline methods: the three components from device points, 1-D fields on z, 2-D on x, y
+        f.device_spline().eval_device(pts_dev, m, out, stride=3, col0=2 if f._dim == 1 else 0)
+        _lib.call("pp_scale_offset_batch", out.ptr, float(f._scaling), 1, 0.0, 0, m, out.ptr, None)
         return out
     if isinstance(f, Field):
         # components on different grids (or custom callables): one device gather per component

@@ -27,7 +27,7 @@ import numpy as np
 
 from . import _lib
 from .batch import BATCH_ALGORITHMS, make_map
-from .device import DeviceArray, mem_info, synchronize
+from .device import DeviceArray, Event, PinnedArray, copy_d2h_async, copy_h2d_async, mem_info, synchronize
 from .engine import DEFAULT_REFINE_BITS, STAGGERED, U_STATE, ParticleState, motion_axis
 from .interpolators import DeviceFieldMap
 
@@ -106,12 +106,71 @@ class _FreeList:
         return sum(c for _, c in self.ranges)
 
 
+class _PinnedPool:
+    """Reusable page-locked host buffers (admission uploads, result downloads)."""
+
+    def __init__(self):
+        self.free = []
+
+    def take(self, nbytes):
+        best = None
+        for k, b in enumerate(self.free):
+            if b.nbytes >= nbytes and (best is None or b.nbytes < self.free[best].nbytes):
+                best = k
+        if best is not None:
+            return self.free.pop(best)
+        return PinnedArray(max(nbytes, 1 << 20))
+
+    def give(self, buf):
+        self.free.append(buf)
+
+
+class _RoundSlot:
+    """Device and pinned tile tables of one round in flight, reusable once its event has completed."""
+
+    def __init__(self):
+        self.event = Event()
+        self.pending = False
+        self.table_pin = self.tiles_pin = None
+        self.table_dev = self.tb_dev = self.ts_dev = None
+
+    def ensure(self, nbunch, ntiles):
+        tb_bytes = ctypes.sizeof(_lib.PPBunch) * max(nbunch, 1)
+        tl_bytes = 12 * max(ntiles, 1)          # int32 bunch + int64 start per tile
+        if self.table_pin is None or self.table_pin.nbytes < tb_bytes:
+            self.table_pin = PinnedArray(2 * tb_bytes)
+            self.table_dev = DeviceArray((2 * tb_bytes,), np.uint8)
+        if self.tiles_pin is None or self.tiles_pin.nbytes < tl_bytes:
+            n = 2 * max(ntiles, 1)             # even: the int64 starts begin 8-byte aligned at 4 n bytes
+            self.tiles_pin = PinnedArray(12 * n)
+            self.tiles_cap = n
+            self.tb_dev = DeviceArray((n,), np.int32)
+            self.ts_dev = DeviceArray((n,), np.int64)
+
+    def tile_views(self, ntiles):
+        """(tile_bunch int32, tile_start int64) views of the pinned tile table; starts at byte 4 * capacity."""
+        return (self.tiles_pin.view(np.int32, ntiles),
+                self.tiles_pin.view(np.int64, ntiles, offset_bytes=4 * self.tiles_cap))
+
+
 class TrackingServer:
+    """Continuous batching with asynchronous rounds.
+
+    Every device operation is enqueued on the library's stream and the host does
+    not wait for it: admissions upload through page-locked buffers, each round's
+    tile tables go to per-round device buffers (``max_inflight`` rounds may be
+    queued; the host blocks only to reuse the buffers of the round that many
+    rounds back), and finished jobs are downloaded asynchronously and completed
+    when an event recorded after their copies has passed.  Pool slots are
+    freed at enqueue time: later work on the same stream runs after the
+    retirement copies.
+    """
+
     def __init__(self, maps, precision="fp32_mixed", chunk_steps=256, pool_particles=None, hbm_fraction=0.5,
-                 block_size=256, sort_every_chunks=4, max_bypass=8):
+                 block_size=256, sort_every_chunks=4, max_bypass=8, max_inflight=2):
         if precision not in ("fp32_mixed", "fp64_fast"):
             raise ValueError("serving runs in 'fp32_mixed' or 'fp64_fast'")
-        self.maps = [m if isinstance(m, DeviceFieldMap) else make_map(*m) for m in maps]
+        self.maps = [m if isinstance(m, DeviceFieldMap) else make_map(*m, precision=precision) for m in maps]
         if not 1 <= len(self.maps) <= _lib.MAX_BATCH_MAPS:
             raise ValueError(f"1 to {_lib.MAX_BATCH_MAPS} maps")
         self.precision = precision
@@ -129,13 +188,18 @@ class TrackingServer:
         self.free = _FreeList(self.pool_n)
         self.queue = collections.deque()
         self.running = []
+        self.retiring = []          # (job, event, pinned buffer) with downloads in flight
         self.finished = []
         self.lock = threading.Lock()
         self.rounds = 0
         self.particle_steps = 0
         self._ws = None
         self._ws_bytes = 0
-        self._stage = None
+        self._stage = None          # device AoS staging of admissions (2n x 3 doubles: r then v)
+        self._out = None            # device AoS staging of retirements
+        self._pinned = _PinnedPool()
+        self._uploads = []          # (event, pinned buffer) of admission uploads in flight
+        self._slots = [_RoundSlot() for _ in range(max(1, int(max_inflight)))]
 
     # ---- client side -------------------------------------------------------------
     def submit(self, job):
@@ -152,7 +216,7 @@ class TrackingServer:
     @property
     def idle(self):
         with self.lock:
-            return not self.queue and not self.running
+            return not self.queue and not self.running and not self.retiring
 
     # ---- device helpers ------------------------------------------------------------
     def _range_args(self, job):
@@ -168,11 +232,18 @@ class TrackingServer:
         a.efield = t
         return a
 
+    def _grow(self, attr, nbytes):
+        """A device buffer of at least nbytes; growing waits for the stream (the old one may be in use)."""
+        buf = getattr(self, attr)
+        if buf is None or buf.nbytes < nbytes:
+            synchronize()
+            setattr(self, attr, DeviceArray((max(nbytes, 1 << 20),), np.uint8))
+        return getattr(self, attr)
+
     def _workspace(self, n):
-        if self._ws_bytes < 1 or self._ws_need(n) > self._ws_bytes:
-            b = self._ws_need(max(n, 1 << 20))
-            self._ws = DeviceArray((b,), np.uint8)
-            self._ws_bytes = b
+        need = self._ws_need(max(n, 1 << 20))
+        if self._ws is None or self._ws.nbytes < need:
+            self._ws = self._grow("_ws", need)
         return self._ws
 
     @staticmethod
@@ -187,38 +258,74 @@ class TrackingServer:
                   self.pool.ids.view_ptr(job.offset), ws.ptr, ws.nbytes, None)
 
     def _admit(self, job, offset):
+        """Enqueue the job's upload into its pool range (r, v through page-locked memory), ids, u and sort."""
         job.offset = offset
         p, o, n = self.pool, offset, job.n
-        stage = DeviceArray.from_host(job.r0)
+        nb = 48 * n
+        pin = self._pinned.take(nb)
+        h = pin.view(np.float64, 6 * n)
+        h[:3 * n] = job.r0.reshape(-1)
+        h[3 * n:] = job.v0.reshape(-1)
+        stage = self._grow("_stage", nb)
+        copy_h2d_async(stage.ptr, pin, nb)
         _lib.call("pp_aos_to_soa", stage.ptr, n, p.x.view_ptr(o), p.y.view_ptr(o), p.z.view_ptr(o), None)
-        stage.upload(job.v0)
-        _lib.call("pp_aos_to_soa", stage.ptr, n, p.vx.view_ptr(o), p.vy.view_ptr(o), p.vz.view_ptr(o), None)
+        _lib.call("pp_aos_to_soa", stage.ptr + 24 * n, n, p.vx.view_ptr(o), p.vy.view_ptr(o), p.vz.view_ptr(o),
+                  None)
         _lib.call("pp_iota_i32", p.ids.view_ptr(o), n, None)
         if job.algorithm in U_STATE:
             _lib.call("pp_v_to_u", p.vx.view_ptr(o), p.vy.view_ptr(o), p.vz.view_ptr(o), n, None)
         if n >= 4096:
             self._sort_job(job)
-        synchronize()           # the staging buffer is released on return
+        ev = Event()
+        ev.record()
+        self._uploads.append((ev, pin))
         job.state = "running"
         job.t_admit = time.perf_counter()
 
     def _retire(self, job):
+        """Enqueue the job's unsort and download; it completes when its event has passed (_poll)."""
         p, o, n = self.pool, job.offset, job.n
         ws = self._workspace(n)
         _lib.call("pp_unsort_state", self._range_args(job), p.ids.view_ptr(o), ws.ptr, ws.nbytes, None)
         if job.algorithm in U_STATE:
             _lib.call("pp_u_to_v", p.vx.view_ptr(o), p.vy.view_ptr(o), p.vz.view_ptr(o), n, None)
-        out = DeviceArray((n, 3))
+        nb = 48 * n
+        out = self._grow("_out", nb)
         _lib.call("pp_soa_to_aos", p.x.view_ptr(o), p.y.view_ptr(o), p.z.view_ptr(o), n, out.ptr, None)
-        r = out.to_host()
-        _lib.call("pp_soa_to_aos", p.vx.view_ptr(o), p.vy.view_ptr(o), p.vz.view_ptr(o), n, out.ptr, None)
-        v = out.to_host()
-        job.result = (r, v)
-        job.t_done = time.perf_counter()
-        job.state = "done"
-        self.free.free(o, n)
+        _lib.call("pp_soa_to_aos", p.vx.view_ptr(o), p.vy.view_ptr(o), p.vz.view_ptr(o), n, out.ptr + 24 * n, None)
+        pin = self._pinned.take(nb)
+        copy_d2h_async(pin, out.ptr, nb)
+        ev = Event()
+        ev.record()
+        self.retiring.append((job, ev, pin))
+        job.state = "retiring"
+        self.free.free(o, n)          # later work on the stream runs after these copies
         job.offset = None
-        job.event.set()
+
+    def _poll(self, block=False):
+        """Complete jobs whose downloads have finished; release upload buffers.  block: wait for the oldest."""
+        if block and self.retiring:
+            self.retiring[0][1].sync()
+        still = []
+        for job, ev, pin in self.retiring:
+            if ev.query():
+                h = pin.view(np.float64, 6 * job.n)
+                job.result = (h[:3 * job.n].reshape(job.n, 3).copy(), h[3 * job.n:].reshape(job.n, 3).copy())
+                job.t_done = time.perf_counter()
+                job.state = "done"
+                self._pinned.give(pin)
+                self.finished.append(job)
+                job.event.set()
+            else:
+                still.append((job, ev, pin))
+        self.retiring = still
+        keep = []
+        for ev, pin in self._uploads:
+            if ev.query():
+                self._pinned.give(pin)
+            else:
+                keep.append((ev, pin))
+        self._uploads = keep
 
     # ---- scheduling ------------------------------------------------------------------
     def _admit_queued(self, now=None):
@@ -243,46 +350,59 @@ class TrackingServer:
         return admitted
 
     def round(self, now=None):
-        """One scheduling round: admit, advance all running jobs one chunk, retire finished ones."""
+        """One scheduling round: admit, enqueue one chunk of every running job, enqueue the retirement of
+        jobs that finish with it, and complete jobs whose downloads have arrived.  Returns the work enqueued."""
         self._admit_queued(now)
         if not self.running:
+            self._poll(block=bool(self.retiring))
             return 0
-        table = (_lib.PPBunch * len(self.running))()
+        slot = self._slots[self.rounds % len(self._slots)]
+        if slot.pending:
+            slot.event.sync()           # its tables are about to be rewritten
+            slot.pending = False
+        nb = len(self.running)
+        nt = ctypes.c_int64()
+        slot.ensure(nb, 0)
+        table = (_lib.PPBunch * nb).from_address(slot.table_pin.ptr)
         work = 0
-        for k, job in enumerate(self.running):
+        # tiles of the costliest jobs first (four gathers per RK4 step): the dispatcher hands out
+        # workgroups in table order, so the launch does not end on a tail of long tiles
+        order = sorted(self.running, key=lambda j: -(4 if j.algorithm in ("rk4", "rk4_rel") else 1)
+                       * min(self.chunk_steps, j.nsteps - j.done_steps))
+        for k, job in enumerate(order):
             t = table[k]
             t.offset, t.count = job.offset, job.n
             t.algorithm = _lib.ALG_IDS[job.algorithm]
             t.nsteps = min(self.chunk_steps, job.nsteps - job.done_steps)
             t.map_index = job.map_index
+            t.flags = 0
             if job.algorithm in STAGGERED:     # track_batch's half kicks in the first and last chunk
                 t.flags = ((_lib.BUNCH_KICK_START if job.done_steps == 0 else 0)
                            | (_lib.BUNCH_KICK_END if job.done_steps + t.nsteps == job.nsteps else 0))
             t.dt, t.q_over_m, t.escale, t.bscale = job.dt, job.q_over_m, job.escale, job.bscale
             work += job.n * t.nsteps
-        nt = ctypes.c_int64()
-        _lib.call("pp_batch_tiles", ctypes.addressof(table), len(table), self.block_size, None, None,
+        _lib.call("pp_batch_tiles", slot.table_pin.ptr, nb, self.block_size, None, None, ctypes.byref(nt))
+        ntiles = max(nt.value, 1)
+        slot.ensure(nb, ntiles)
+        table = (_lib.PPBunch * nb).from_address(slot.table_pin.ptr)
+        tb, ts = slot.tile_views(ntiles)
+        _lib.call("pp_batch_tiles", slot.table_pin.ptr, nb, self.block_size, tb.ctypes.data, ts.ctypes.data,
                   ctypes.byref(nt))
-        tb = np.empty(max(nt.value, 1), dtype=np.int32)
-        ts = np.empty(max(nt.value, 1), dtype=np.int64)
-        _lib.call("pp_batch_tiles", ctypes.addressof(table), len(table), self.block_size, tb.ctypes.data,
-                  ts.ctypes.data, ctypes.byref(nt))
-        dtab = DeviceArray((ctypes.sizeof(table),), np.uint8)
-        _lib.call("pp_memcpy_h2d", dtab.ptr, ctypes.addressof(table), ctypes.sizeof(table), None)
-        dtb, dts = DeviceArray.from_host(tb), DeviceArray.from_host(ts)
+        copy_h2d_async(slot.table_dev.ptr, slot.table_pin, ctypes.sizeof(_lib.PPBunch) * nb)
+        copy_h2d_async(slot.tb_dev.ptr, slot.tiles_pin, 4 * ntiles)
+        copy_h2d_async(slot.ts_dev.ptr, slot.tiles_pin, 8 * ntiles, offset_bytes=4 * slot.tiles_cap)
         a = _lib.PPBatchArgs()
         p = self.pool
         a.precision = _lib.PRECISIONS[self.precision]
         a.block_size = self.block_size
         a.n = self.pool_n
         a.x, a.y, a.z, a.vx, a.vy, a.vz = p.x.ptr, p.y.ptr, p.z.ptr, p.vx.ptr, p.vy.ptr, p.vz.ptr
-        a.bunches, a.nbunch, a.nmaps = dtab.ptr, len(table), len(self.maps)
-        a.tile_bunch, a.tile_start, a.ntiles = dtb.ptr, dts.ptr, nt.value
+        a.bunches, a.nbunch, a.nmaps = slot.table_dev.ptr, nb, len(self.maps)
+        a.tile_bunch, a.tile_start, a.ntiles = slot.tb_dev.ptr, slot.ts_dev.ptr, nt.value
         a.step0, a.nsteps = 0, self.chunk_steps
         for k, m in enumerate(self.maps):
             a.maps[k] = m.handle
         _lib.call("pp_push_batched", ctypes.byref(a), None)
-        synchronize()
         self.rounds += 1
         self.particle_steps += work
         still = []
@@ -291,12 +411,14 @@ class TrackingServer:
             job.chunks += 1
             if job.done_steps >= job.nsteps:
                 self._retire(job)
-                self.finished.append(job)
             else:
                 if self.sort_every_chunks and job.chunks % self.sort_every_chunks == 0 and job.n >= 4096:
                     self._sort_job(job)
                 still.append(job)
         self.running = still
+        slot.event.record()
+        slot.pending = True
+        self._poll()
         return work
 
     def run_until_idle(self, max_rounds=10 ** 9):
@@ -304,6 +426,8 @@ class TrackingServer:
         while not self.idle and r < max_rounds:
             self.round(time.perf_counter())
             r += 1
+        synchronize()
+        self._poll()
 
 
 def c5_jobs(n_jobs, rate, seed=5, nmin=1e3, nmax=2e6, smin=1e2, smax=2e4, t0=0.0, template=None):

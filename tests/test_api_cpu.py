@@ -81,8 +81,18 @@ def test_field_validation_without_device():
         pa.Field(units="furlong")
     with pytest.raises(ValueError, match="Unknown backend"):
         pa.Field.from_arrays({"x": [0.0, 1.0]}, {"x": np.zeros(2)}, interpolator_backend="magic")
+    # CoordinateMapper's methods and its error text (interpolators.py:450-460)
+    with pytest.raises(ValueError, match="^Method 'pchip' not supported$"):
+        pa.get_interpolator((np.arange(3.0),), np.zeros(3), method="pchip")
+    for method, order in (("nearest", 0), ("linear", 1), ("cubic", 3), ("quintic", 5)):
+        it = pa.get_interpolator((np.arange(3.0),), np.zeros(3), method=method)
+        assert it.order == order
+        # the per-call override knows nearest / linear / cubic; other names keep the order (:482-485)
+        assert it.call_order("cubic") == 3 and it.call_order("quintic") == order and it.call_order(None) == order
+    # RegularGridInterpolator's spline methods are not served on the device
     with pytest.raises(NotImplementedError):
-        pa.get_interpolator((np.arange(3.0),), np.zeros(3), method="cubic")
+        pa.get_interpolator((np.arange(3.0),), np.zeros(3), method="cubic", backend="scipy")
+    assert pa.get_interpolator((np.arange(3.0),), np.zeros(3), method="nearest", backend="scipy").order == 0
     f = pa.Field.from_arrays({"x": np.arange(3.0), "y": [0.0], "z": np.arange(4.0)},
                              {c: np.zeros((3, 4)) for c in "xyz"})
     assert f.dim == 2

@@ -151,3 +151,25 @@ def test_server_job_alone_equals_in_the_mix(maps):
         bitwise(j.result[0], r)
         bitwise(j.result[1], v)
         assert j.latency > 0
+
+
+def test_autotuner_picks_a_candidate_and_results_do_not_depend_on_it(maps, monkeypatch):
+    from pypatools_amd import autotune
+    grid, f1, f2, fields = maps
+    monkeypatch.delenv("PYPAT_AUTOTUNE_CACHE", raising=False)
+    rng = np.random.default_rng(35)
+    bunches = _bunches(rng, [5000, 700, 3000, 1500], ["boris", "vay", "rk4", "boris"])
+    cfg = autotune.autotune_batch(fields, bunches, "fp32_mixed", sample_steps=120, use_cache=True)
+    assert cfg["block_size"] in autotune.BLOCKS and cfg["steps_per_launch"] in autotune.STEPS_PER_LAUNCH
+    assert cfg["sort_every"] in autotune.SORT_EVERY
+    # the second call comes from the per-process cache (no file under $HOME)
+    t = __import__("time").perf_counter()
+    assert autotune.autotune_batch(fields, bunches, "fp32_mixed", sample_steps=120) == cfg
+    assert __import__("time").perf_counter() - t < 0.05
+    tuned = BatchedTracker(fields, "fp32_mixed", **cfg).load(bunches)
+    tuned.run()
+    default = BatchedTracker(fields, "fp32_mixed", **autotune.DEFAULT).load(bunches)
+    default.run()
+    for (r, v), (rr, vv) in zip(tuned.results(), default.results()):
+        bitwise(r, rr)
+        bitwise(v, vv)
