@@ -367,6 +367,18 @@ int pp_beam_moments(const pp_push_args* state, int32_t u_state, double mass_mev,
    eps_x, eps_y; counts_dev[2]. */
 int pp_beam_ellipse_counts(const pp_push_args* state, int32_t u_state, const double* twiss_dev, double* counts_dev,
                            void* workspace, size_t ws_bytes, pp_stream s);
+/* pp_beam_moments without a workspace and without any allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The block partials go to a
+   fixed device scratch of pp_beam_noalloc_scratch_bytes() (147456 B: 1024 blocks x 18
+   doubles) that each device keeps from pp_init until the process ends; any particle count
+   fits in it.  Same kernels and results as pp_beam_moments.  pp_init must have been called
+   (PP_ERR_INVALID otherwise).  Every call on one device shares that device's scratch and
+   leaves its contents behind: calls that overlap on one device (two streams, two threads)
+   race and must be ordered by the caller. */
+int pp_beam_moments_noalloc(const pp_push_args* state, int32_t u_state, double mass_mev, int32_t z_energy,
+                            double* sums_dev, double* central_dev, pp_stream s);
+int pp_beam_noalloc_scratch_bytes(size_t* bytes);
 
 /* Electrode surfaces as a triangle mesh for particle loss (replaces the
    electrode_assembly.segment_intersects_surface call of Pusher.track_batch,

@@ -185,6 +185,9 @@ SIGNATURES = {
                                   c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "pp_beam_ellipse_counts": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, c_void_p, c_void_p, c_void_p,
                                          ctypes.c_size_t, c_void_p]),
+    "pp_beam_moments_noalloc": (c_int32, [ctypes.POINTER(PPPushArgs), c_int32, ctypes.c_double, c_int32, c_void_p,
+                                          c_void_p, c_void_p]),
+    "pp_beam_noalloc_scratch_bytes": (c_int32, [ctypes.POINTER(ctypes.c_size_t)]),
     "pp_soa_to_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "pp_v_to_u": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "pp_u_to_v": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),

@@ -98,6 +98,8 @@ int pp_init(int device) {
         HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
         g_device = device;
     }
+    // device address of the fixed scratch of pp_beam_moments_noalloc, resolved once per device
+    HIP_TRY(pp_beam::init_noalloc_scratch(device), "hipGetSymbolAddress(scratch)");
     return PP_OK;
 }
 
@@ -1204,6 +1206,29 @@ int pp_beam_moments(const pp_push_args* state, int32_t u_state, double mass_mev,
     if (!sums_dev || !central_dev || !workspace) return fail(PP_ERR_INVALID, "null output or workspace");
     if (ws_bytes < pp_beam::workspace_bytes()) return fail(PP_ERR_INVALID, "workspace too small");
     HIP_TRY(pp_beam::moments(m, sums_dev, central_dev, workspace, stream_of(s)), "beam moment kernels");
+    return PP_OK;
+}
+
+int pp_beam_moments_noalloc(const pp_push_args* state, int32_t u_state, double mass_mev, int32_t z_energy,
+                            double* sums_dev, double* central_dev, pp_stream s) {
+    // no allocation of any kind here: the library must already be initialised on this thread's device
+    if (g_device < 0) return fail(PP_ERR_INVALID, "pp_beam_moments_noalloc: call pp_init first");
+    MomentArgs m;
+    int rc = moment_args(state, u_state, mass_mev, z_energy, m);
+    if (rc) return rc;
+    if (!sums_dev || !central_dev) return fail(PP_ERR_INVALID, "null output");
+    if (pp_beam::workspace_bytes() > pp_beam::noalloc_scratch_bytes())
+        return fail(PP_ERR_UNSUPPORTED, "pp_beam_moments_noalloc: %zu B of block partials exceed the %zu B scratch",
+                    pp_beam::workspace_bytes(), pp_beam::noalloc_scratch_bytes());
+    void* ws = pp_beam::noalloc_scratch(g_device);
+    if (!ws) return fail(PP_ERR_INVALID, "pp_beam_moments_noalloc: no scratch on device %d (call pp_init)", g_device);
+    HIP_TRY(pp_beam::moments(m, sums_dev, central_dev, ws, stream_of(s)), "beam moment kernels");
+    return PP_OK;
+}
+
+int pp_beam_noalloc_scratch_bytes(size_t* bytes) {
+    if (!bytes) return fail(PP_ERR_INVALID, "null argument");
+    *bytes = pp_beam::noalloc_scratch_bytes();
     return PP_OK;
 }
 

@@ -206,6 +206,10 @@ namespace pp_beam {
 size_t workspace_bytes();
 hipError_t moments(const MomentArgs& a, double* sums, double* central, void* ws, hipStream_t s);
 hipError_t ellipse_counts(const MomentArgs& a, const double* twiss_dev, double* counts, void* ws, hipStream_t s);
+// fixed per-device scratch of pp_beam_moments_noalloc (see pp_moments.hip)
+size_t noalloc_scratch_bytes();
+hipError_t init_noalloc_scratch(int device);   // with `device` current; once per device
+void* noalloc_scratch(int device);             // device address, or null before init_noalloc_scratch
 }  // namespace pp_beam
 
 // box mesh of the space-charge solve: n cell-centre nodes per axis, spacing h = L / n

@@ -8,6 +8,8 @@
 // Each pass writes one partial per block; one block per quantity then combines
 // the partials in a fixed order, so a result does not depend on scheduling.  A third pass counts
 // the particles inside the 4-rms Twiss ellipses.
+#include <mutex>
+
 #include "pp_internal.h"
 
 namespace pp_beamk {
@@ -197,6 +199,43 @@ static int blocks_for(int64_t n) {
 }
 
 size_t workspace_bytes() { return sizeof(double) * (size_t)kMomentBlocks * pp_beamk::kQ1; }
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch for the allocation-free moments (pp_beam_moments_noalloc):
+// the block partials of one call, kMomentBlocks x kQ1 doubles (147456 B), held by
+// every device the library initialises until the process ends.  In host code the
+// array's name is only a host-side stand-in; kernels get the device address that
+// hipGetSymbolAddress returns for the current device, resolved once per device.
+// Every call on one device shares its array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) double scratch[(size_t)kMomentBlocks * pp_beamk::kQ1];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+size_t noalloc_scratch_bytes() { return sizeof(scratch); }
+
+hipError_t init_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 hipError_t moments(const MomentArgs& a, double* sums, double* central, void* ws, hipStream_t s) {
     double* partial = static_cast<double*>(ws);
