@@ -579,6 +579,7 @@ def run_c5(args, n_jobs=None):
     """C5: Poisson arrivals, continuous batching, p50/p99 latency at a load set from measured capacity.
     Returns (jobs, lam, capacity, measured, span, srv)."""
     import pypatools_amd as pa
+    from pypatools_amd.device import synchronize
     from pypatools_amd.serving import TrackingServer, c5_jobs
     from pypatools_amd.workloads import c2_map
     grid, E, B = c2_map(128)
@@ -594,14 +595,25 @@ def run_c5(args, n_jobs=None):
     measured = None
     if capacity <= 0:
         # capacity of the server on the C5 job mix (Boris / Vay / RK4, which costs 4 gathers per step):
-        # a closed batch of C5-distributed jobs submitted at once and run to completion
-        cal = c5_jobs(max(args.jobs // 5, 40), 1e9, seed=11)
+        # a closed batch of C5-distributed jobs submitted at once, timed until the queue has emptied and
+        # fewer than 8 jobs are left running (the drain of the last few long jobs is not capacity)
+        cal = c5_jobs(max(args.jobs // 2, 60), 1e9, seed=11)
+        warm = c5_jobs(4, 1e9, seed=12)          # first admissions allocate the page-locked buffers
+        for j in warm:
+            srv.submit(j)
+        srv.run_until_idle()
+        srv.particle_steps = 0
         t = time.perf_counter()
         for j in cal:
             srv.submit(j)
-        srv.run_until_idle()
-        el = time.perf_counter() - t
-        measured = sum(j.n * j.nsteps for j in cal) / el
+        el, done = None, 0
+        while not srv.idle:
+            srv.round(time.perf_counter())
+            if el is None and not srv.queue and len(srv.running) < 8:
+                synchronize()
+                el = time.perf_counter() - t
+                done = srv.particle_steps          # every enqueued chunk has completed
+        measured = done / el
         capacity = measured
         srv.particle_steps = 0
         srv.rounds = 0

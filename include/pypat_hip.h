@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define PP_ABI_VERSION 2
+#define PP_ABI_VERSION 3
 
 enum pp_status {
     PP_OK = 0,
@@ -299,7 +299,15 @@ typedef struct pp_batch_args {
     int64_t nsteps;
     const int64_t* step0_dev;  /* optional device scalar read instead of step0 (graph replays) */
     const pp_map* maps[PP_MAX_BATCH_MAPS];   /* 6-component (E then B) uniform 3-D maps */
+    int32_t tile_class;      /* enum pp_tile_class: which algorithms the launch's kernel holds; tiles of
+                                bunches outside the class are skipped (their state is left as it is) */
+    int32_t _pad;
 } pp_batch_args;
+/* PP_TILES_ALL: one kernel for every algorithm (allocated for 4 wavefronts per SIMD);
+   PP_TILES_SINGLE_GATHER: boris, vay, vay_rel, boris_rel, leapfrog (6 wavefronts per SIMD);
+   PP_TILES_RK4: rk4, rk4_rel (4 wavefronts per SIMD).  Callers with both kinds of bunches pass
+   each launch the tiles of one class. */
+enum pp_tile_class { PP_TILES_ALL = 0, PP_TILES_SINGLE_GATHER = 1, PP_TILES_RK4 = 2 };
 
 /* Host helper: split the bunches into tiles of `tile` particles that never
    straddle two bunches.  tile_bunch/tile_start: host arrays with room for

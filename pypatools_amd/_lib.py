@@ -61,6 +61,8 @@ class PPBunch(ctypes.Structure):
 
 
 SPLINE_NEAREST_MODE, SPLINE_RGI = 1, 2          # pp_spline_flags
+TILES_ALL, TILES_SINGLE_GATHER, TILES_RK4 = 0, 1, 2   # pp_tile_class
+RK4_ALGS = ("rk4", "rk4_rel")
 MAX_BATCH_MAPS = 4
 BUNCH_KICK_START, BUNCH_KICK_END = 1, 2          # pp_bunch_flags
 MOMENT_SUMS, MOMENT_CENTRAL = 18, 11          # include/pypat_hip.h PP_MOMENT_SUMS, PP_MOMENT_CENTRAL
@@ -73,7 +75,7 @@ class PPBatchArgs(ctypes.Structure):
         ("bunches", c_void_p), ("nbunch", c_int32), ("nmaps", c_int32),
         ("tile_bunch", c_void_p), ("tile_start", c_void_p), ("ntiles", c_int64),
         ("step0", c_int64), ("nsteps", c_int64), ("step0_dev", c_void_p),
-        ("maps", c_void_p * MAX_BATCH_MAPS),
+        ("maps", c_void_p * MAX_BATCH_MAPS), ("tile_class", c_int32), ("_pad", c_int32),
     ]
 
 

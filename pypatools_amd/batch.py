@@ -57,6 +57,22 @@ class Bunch:
         return self.r0.shape[0]
 
 
+def class_order(tb, ts, bunches, algorithm_of=lambda b: b.algorithm):
+    """Tiles reordered RK4 bunches' first, then the single-gather ones (a stable partition), and the
+    runs (pp_tile_class, first tile, count) of the classes present."""
+    if len(tb) == 0:
+        return tb, ts, []
+    rk4 = np.array([algorithm_of(b) in _lib.RK4_ALGS for b in bunches], dtype=bool)[tb]
+    order = np.concatenate([np.flatnonzero(rk4), np.flatnonzero(~rk4)])
+    n_rk4 = int(rk4.sum())
+    runs = []
+    if n_rk4:
+        runs.append((_lib.TILES_RK4, 0, n_rk4))
+    if n_rk4 < len(tb):
+        runs.append((_lib.TILES_SINGLE_GATHER, n_rk4, len(tb) - n_rk4))
+    return np.ascontiguousarray(tb[order]), np.ascontiguousarray(ts[order]), runs
+
+
 def make_map(efield, bfield, precision=None):
     """The fused 6-component device map of an (efield, bfield) pair on one uniform 3-D grid
     (for precision 'fp32_mixed' without the FP64 coefficient table)."""
@@ -114,8 +130,9 @@ class BatchedTracker:
         _lib.call("pp_batch_tiles", ctypes.addressof(table), len(self.bunches), self.block_size, tb.ctypes.data,
                   ts.ctypes.data, ctypes.byref(nt))
         self.ntiles = nt.value
-        self.tile_bunch = DeviceArray.from_host(tb)
-        self.tile_start = DeviceArray.from_host(ts)
+        tb, ts, self.tile_runs = class_order(tb[:self.ntiles], ts[:self.ntiles], self.bunches)
+        self.tile_bunch = DeviceArray.from_host(tb if self.ntiles else np.zeros(1, np.int32))
+        self.tile_start = DeviceArray.from_host(ts if self.ntiles else np.zeros(1, np.int64))
         r = np.concatenate([b.r0 for b in self.bunches]) if self.bunches else np.zeros((0, 3))
         v = np.concatenate([b.v0 for b in self.bunches]) if self.bunches else np.zeros((0, 3))
         from .engine import ParticleState
@@ -142,7 +159,13 @@ class BatchedTracker:
         self.step = 0
         return self
 
-    def _args(self, step0, nsteps, step0_dev=None):
+    def _launch(self, step0, nsteps, stream, step0_dev=None):
+        """One pp_push_batched per tile class present (RK4 tiles first: they are the longest)."""
+        for cls, first, count in self.tile_runs:
+            _lib.call("pp_push_batched", ctypes.byref(self._args(step0, nsteps, step0_dev, (cls, first, count))),
+                      stream)
+
+    def _args(self, step0, nsteps, step0_dev=None, run=None):
         st = self.state
         a = _lib.PPBatchArgs()
         a.precision = _lib.PRECISIONS[self.precision]
@@ -152,7 +175,9 @@ class BatchedTracker:
         a.bunches = self.table.ptr
         a.nbunch = len(self.bunches)
         a.nmaps = len(self.maps)
-        a.tile_bunch, a.tile_start, a.ntiles = self.tile_bunch.ptr, self.tile_start.ptr, self.ntiles
+        cls, first, count = run if run is not None else (_lib.TILES_ALL, 0, self.ntiles)
+        a.tile_bunch, a.tile_start = self.tile_bunch.view_ptr(first), self.tile_start.view_ptr(first)
+        a.ntiles, a.tile_class = count, cls
         a.step0, a.nsteps = int(step0), int(nsteps)
         a.step0_dev = step0_dev.ptr if step0_dev is not None else None
         for k, m in enumerate(self.maps):
@@ -178,7 +203,7 @@ class BatchedTracker:
         t0 = time.perf_counter()
         if self.max_steps == 0 and self.step == 0:
             # only 0-step bunches: one window [0, 1) gives the staggered ones both half kicks
-            _lib.call("pp_push_batched", ctypes.byref(self._args(0, 1)), stream)
+            self._launch(0, 1, stream)
             self.step = -1
         if self.use_graph:
             self._run_graph(end)
@@ -191,7 +216,7 @@ class BatchedTracker:
                 k = min(spl, end - self.step)
                 if self.sort_every > 0:
                     k = min(k, self.sort_every - since)
-                _lib.call("pp_push_batched", ctypes.byref(self._args(self.step, k)), stream)
+                self._launch(self.step, k, stream)
                 self.step += k
                 if self.sort_every > 0:
                     since += k
@@ -215,7 +240,7 @@ class BatchedTracker:
             if k not in graphs:
                 _lib.call("pp_graph_begin", s.handle)
                 try:
-                    _lib.call("pp_push_batched", ctypes.byref(self._args(0, k, counter)), s.handle)
+                    self._launch(0, k, s.handle, counter)
                     _lib.call("pp_add_i64", counter.ptr, k, s.handle)
                 finally:
                     g = ctypes.c_void_p()

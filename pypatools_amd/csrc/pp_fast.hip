@@ -129,6 +129,16 @@ struct GenericFE32 {
     }
 };
 
+// Field evaluators with raw_e() (kRawE) hand E over before its multiplier.
+template <class FE, class = void>
+struct HasRawE {
+    static constexpr bool value = false;
+};
+template <class FE>
+struct HasRawE<FE, decltype((void)FE::kRawE)> {
+    static constexpr bool value = FE::kRawE;
+};
+
 // PRE: the field evaluator returns h E and h B (h = q/m dt / 2) for the
 // Boris-family steps, folding the scale into its field multipliers.
 template <int ALG, bool PRE = false, class FE>
@@ -163,6 +173,19 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
         const f3 ksum = (k1v + 2.f * k2v) + (2.f * k3v + k4v);
         v = v + w * tod(ksum);
         if constexpr (REL) v = clamp_speed(v, p.vclamp);
+        vrec = v;
+    } else if constexpr (ALG == PP_ALG_BORIS && PRE && HasRawE<FE>::value) {
+        // as below, with E's multiplier kE (h times its scale) folded into two FMAs per component
+        float kE;
+        fe.raw_e(i, r.x, r.y, r.z, gs, 0, E, B, kE);
+        const f3 vf = tof(v);
+        const f3 vmf = mkf(fmaf(kE, E.x, vf.x), fmaf(kE, E.y, vf.y), fmaf(kE, E.z, vf.z));
+        const f3 s = (2.f * rcpf(1.f + norm2f(B))) * B;
+        const f3 vp = vmf + crossf(vmf, B);
+        const f3 c = crossf(vp, s);
+        const float k2 = 2.f * kE;
+        v = addf(v, mkf(fmaf(k2, E.x, c.x), fmaf(k2, E.y, c.y), fmaf(k2, E.z, c.z)));
+        r = r + v * dt;
         vrec = v;
     } else {
         fe(i, r.x, r.y, r.z, gs, 0, E, B);
@@ -545,6 +568,22 @@ DEV void stage_cells(float4* wv, uint32_t* list, __amdgpu_buffer_rsrc_t rs, uint
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The wavefront's LDS slots and cell list.  One area per element size in each kernel, shared by the
+// gather functors a kernel instantiates (a staging with reuse and the kicks outside its step loop
+// never interleave).
+DEV float4* stage_area32() {
+    __shared__ float4 lds[kStageWaves][kStageSlots * StageLanes<12>::kPitch];
+    return lds[threadIdx.x >> 6];
+}
+DEV float4* stage_area64() {
+    __shared__ float4 lds[kStageWaves][kStageSlots * StageLanes<24>::kPitch];
+    return lds[threadIdx.x >> 6];
+}
+DEV uint32_t* stage_list() {
+    __shared__ uint32_t list[kStageWaves][kStageSlots];
+    return list[threadIdx.x >> 6];
+}
+
 // Constant-address-space view of the coefficient table: loads at a wave-uniform
 // address become scalar loads (s_load_dwordx16) into SGPRs.
 typedef const __attribute__((address_space(4))) float* cfloat_p;
@@ -605,19 +644,46 @@ struct HotFE32 {
     mutable uint32_t last_cell = 0;   // REUSE: this lane's cell and LDS slot at the last staging
     mutable uint32_t last_slot = 0;
     mutable bool staged = false;      // REUSE: the wavefront has staged at least once (wave-uniform)
+    static constexpr bool kRawE = true;   // raw_e(): E before its multiplier (step_mixed<BORIS> folds it)
+
+    // field multipliers at global step gs and RK stage (RF factors applied)
+    DEV void scales(int64_t gs, int stage, float& kE, float& kB) const {
+        kE = k.kE;
+        kB = k.kB;
+        if constexpr (RF) {
+            if (rf.e != PP_RF_NONE) kE *= (float)hot_rf(rf, rf.e, gs, stage);
+            if (rf.b != PP_RF_NONE) kB *= (float)hot_rf(rf, rf.b, gs, stage);
+        }
+    }
+
     DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B) const {
+        float v[6], kE, kB;
+        gather(x, y, z, v);
+        scales(gs, stage, kE, kB);
+        E = mkf(kE * v[0], kE * v[1], kE * v[2]);
+        B = mkf(kB * v[3], kB * v[4], kB * v[5]);
+    }
+
+    // E unscaled with its multiplier kE, B scaled
+    DEV void raw_e(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B, float& kE) const {
+        float v[6], kB;
+        gather(x, y, z, v);
+        scales(gs, stage, kE, kB);
+        E = mkf(v[0], v[1], v[2]);
+        B = mkf(kB * v[3], kB * v[4], kB * v[5]);
+    }
+
+    // the six map components (E then B) at a point, unscaled
+    DEV void gather(double x, double y, double z, float (&v)[6]) const {
         double tx, ty, tz;
         bool ok;
         uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
         cell = ok ? cell : kOutside32;
         const Mono w = monomials((float)tx, (float)ty, (float)tz);
-        float v[6];
         bool done = false;
 #if defined(__HIP_DEVICE_COMPILE__)
         constexpr int kPitch = StageLanes<12>::kPitch;
-        __shared__ float4 stage_lds[kStageWaves][kStageSlots * kPitch];
-        __shared__ uint32_t stage_list[kStageWaves][kStageSlots];
-        float4* wv = stage_lds[threadIdx.x >> 6];
+        float4* wv = stage_area32();
         if constexpr (REUSE) {
             if (staged && __builtin_amdgcn_ballot_w64(cell != last_cell) == 0) {
                 const float4* q = wv + __umul24(last_slot, (uint32_t)kPitch);
@@ -645,7 +711,7 @@ struct HotFE32 {
             uint32_t myslot = 0;
             const int ns = distinct_cells(cell, myslot);
             if (ns > 0) {
-                stage_cells<12>(wv, stage_list[threadIdx.x >> 6], rs, 192u, ns, cell, myslot, lanes);
+                stage_cells<12>(wv, stage_list(), rs, 192u, ns, cell, myslot, lanes);
                 const float4* q = wv + __umul24(myslot, (uint32_t)kPitch);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
@@ -665,13 +731,6 @@ struct HotFE32 {
             for (int c = 0; c < 6; ++c)
                 v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
         }
-        float kE = k.kE, kB = k.kB;
-        if constexpr (RF) {
-            if (rf.e != PP_RF_NONE) kE *= (float)hot_rf(rf, rf.e, gs, stage);
-            if (rf.b != PP_RF_NONE) kB *= (float)hot_rf(rf, rf.b, gs, stage);
-        }
-        E = mkf(kE * v[0], kE * v[1], kE * v[2]);
-        B = mkf(kB * v[3], kB * v[4], kB * v[5]);
     }
 };
 
@@ -696,9 +755,7 @@ struct HotFE64 {
         bool done = false;
 #if defined(__HIP_DEVICE_COMPILE__)
         constexpr int kPitch = StageLanes<24>::kPitch;
-        __shared__ float4 stage_lds64[kStageWaves][kStageSlots * kPitch];
-        __shared__ uint32_t stage_list64[kStageWaves][kStageSlots];
-        float4* wv = stage_lds64[threadIdx.x >> 6];
+        float4* wv = stage_area64();
         uint32_t slot = 0;
         if constexpr (REUSE) {
             if (staged && __builtin_amdgcn_ballot_w64(cell != last_cell) == 0) {
@@ -709,7 +766,7 @@ struct HotFE64 {
         if (!done && __builtin_amdgcn_ballot_w64(true) == ~0ull) {
             const int ns = distinct_cells(cell, slot);
             if (ns > 0) {
-                stage_cells<24>(wv, stage_list64[threadIdx.x >> 6], rs, 384u, ns, cell, slot, lanes);
+                stage_cells<24>(wv, stage_list(), rs, 384u, ns, cell, slot, lanes);
                 done = true;
                 if constexpr (REUSE) {
                     last_cell = cell;
@@ -766,8 +823,15 @@ constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || AL
 // the compiler would otherwise spend registers for a deeper load pipeline at 3.
 // Four wavefronts per SIMD also for the RK4 steps: measured on C3, the spills this costs are cheaper than
 // halving the occupancy (2 wavefronts per SIMD ran 1.6x slower without staging reuse).
+// Wavefronts per SIMD the register allocation of k_hot aims for (512 VGPRs per SIMD lane): 6 for the
+// single-gather steps (80 VGPRs; C2 measured 2.06e11 particle-steps/s at 6, 1.95e11 at 5, 1.93e11 at 4,
+// where the 4-wavefront allocation took 116 VGPRs), 4 for the RK4 steps (C3 the same at 4, 5 and 6: their
+// spills grow with the occupancy).
+template <int ALG>
+constexpr int kHotWaves = (ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL) ? 4 : 6;
+
 template <int ALG, bool MIXED, bool PERIODIC, bool RF>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_hot(const HotParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotWaves<ALG>))) k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     const __amdgpu_buffer_rsrc_t rs =
@@ -852,8 +916,21 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
     }
 }
 
-template <bool MIXED, bool PERIODIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_batched(const BatchParams p) {
+// Kernel per tile class (pp_tile_class): the single-gather algorithms alone fit 6 wavefronts per SIMD,
+// like k_hot; a kernel holding the RK4 steps keeps 4 (C4, Boris and Vay bunches only: 1.62e11 particle-steps/s
+// with every algorithm in one kernel at 4, 1.90e11 at 6; the C5 mix with its RK4 jobs ran worse at 6).
+template <int CLASS>
+constexpr int kBatchWaves = CLASS == PP_TILES_SINGLE_GATHER ? 6 : 4;
+
+template <int CLASS>
+constexpr bool batch_class_has(int alg) {
+    return CLASS == PP_TILES_ALL ||
+           (CLASS == PP_TILES_RK4) == (alg == PP_ALG_RK4 || alg == PP_ALG_RK4_REL);
+}
+
+template <bool MIXED, bool PERIODIC, int CLASS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBatchWaves<CLASS>)))
+k_batched(const BatchParams p) {
     const int64_t t = blockIdx.x;
     if (t >= p.ntiles) return;
     const int32_t b = p.tile_bunch[t];
@@ -887,20 +964,47 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     d3 r = mk3(p.x[i], p.y[i], p.z[i]);
     d3 v = mk3(p.vx[i], p.vy[i], p.vz[i]);
     switch (bd.algorithm) {
-        case PP_ALG_BORIS: batch_steps<PP_ALG_BORIS, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end); break;
-        case PP_ALG_VAY: batch_steps<PP_ALG_VAY, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end); break;
+        case PP_ALG_BORIS:
+            if constexpr (batch_class_has<CLASS>(PP_ALG_BORIS))
+                batch_steps<PP_ALG_BORIS, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
+            break;
+        case PP_ALG_VAY:
+            if constexpr (batch_class_has<CLASS>(PP_ALG_VAY))
+                batch_steps<PP_ALG_VAY, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
+            break;
         case PP_ALG_VAY_REL:
-            batch_steps<PP_ALG_VAY_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            if constexpr (batch_class_has<CLASS>(PP_ALG_VAY_REL))
+                batch_steps<PP_ALG_VAY_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
             break;
         case PP_ALG_BORIS_REL:
-            batch_steps<PP_ALG_BORIS_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            if constexpr (batch_class_has<CLASS>(PP_ALG_BORIS_REL))
+                batch_steps<PP_ALG_BORIS_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
             break;
-        case PP_ALG_RK4: batch_steps<PP_ALG_RK4, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end); break;
+        case PP_ALG_RK4:
+            if constexpr (batch_class_has<CLASS>(PP_ALG_RK4))
+                batch_steps<PP_ALG_RK4, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
+            break;
         case PP_ALG_RK4_REL:
-            batch_steps<PP_ALG_RK4_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            if constexpr (batch_class_has<CLASS>(PP_ALG_RK4_REL))
+                batch_steps<PP_ALG_RK4_REL, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
             break;
         case PP_ALG_LEAPFROG:
-            batch_steps<PP_ALG_LEAPFROG, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            if constexpr (batch_class_has<CLASS>(PP_ALG_LEAPFROG))
+                batch_steps<PP_ALG_LEAPFROG, MIXED, PERIODIC>(i, r, v, step0, ns, bd.dt, m, rs, ac, kick_start, kick_end);
+            else
+                return;
             break;
         default: return;
     }
@@ -947,16 +1051,23 @@ hipError_t launch_gather_probe(const float* table, uint32_t bytes, uint32_t ncel
     return hipGetLastError();
 }
 
-hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int block, hipStream_t s) {
+template <int CLASS>
+static void launch_batched_class(bool mixed, bool periodic, const BatchParams& p, dim3 g, dim3 b, hipStream_t s) {
+    if (mixed) {
+        if (periodic) hipLaunchKernelGGL((k_batched<true, true, CLASS>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_batched<true, false, CLASS>), g, b, 0, s, p);
+    } else {
+        if (periodic) hipLaunchKernelGGL((k_batched<false, true, CLASS>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_batched<false, false, CLASS>), g, b, 0, s, p);
+    }
+}
+
+hipError_t launch_batched(int prec, bool periodic, int tile_class, const BatchParams& p, int block, hipStream_t s) {
     const dim3 g((unsigned)p.ntiles), b(block);
     const bool mixed = prec == PP_PREC_FP32_MIXED;
-    if (mixed) {
-        if (periodic) hipLaunchKernelGGL((k_batched<true, true>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_batched<true, false>), g, b, 0, s, p);
-    } else {
-        if (periodic) hipLaunchKernelGGL((k_batched<false, true>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_batched<false, false>), g, b, 0, s, p);
-    }
+    if (tile_class == PP_TILES_SINGLE_GATHER) launch_batched_class<PP_TILES_SINGLE_GATHER>(mixed, periodic, p, g, b, s);
+    else if (tile_class == PP_TILES_RK4) launch_batched_class<PP_TILES_RK4>(mixed, periodic, p, g, b, s);
+    else launch_batched_class<PP_TILES_ALL>(mixed, periodic, p, g, b, s);
     return hipGetLastError();
 }
 

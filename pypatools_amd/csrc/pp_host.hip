@@ -741,7 +741,9 @@ int pp_push_batched(const pp_batch_args* a, pp_stream s) {
     p.vclamp = 0.9999 * kClight;
     p.nmaps = a->nmaps;
     p.nbunch = a->nbunch;
-    HIP_TRY(pp_fast::launch_batched(a->precision, periodic == 1, p, a->block_size, stream_of(s)), "batched launch");
+    if (a->tile_class < PP_TILES_ALL || a->tile_class > PP_TILES_RK4) return fail(PP_ERR_INVALID, "bad tile_class");
+    HIP_TRY(pp_fast::launch_batched(a->precision, periodic == 1, a->tile_class, p, a->block_size, stream_of(s)),
+            "batched launch");
     return PP_OK;
 }
 

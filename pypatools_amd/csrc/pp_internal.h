@@ -154,7 +154,7 @@ hipError_t launch_build_coef(const MapDesc& d, double* c64, float* c32, hipStrea
 namespace pp_fast {
 hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStream_t s);
 hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int block, hipStream_t s);
-hipError_t launch_batched(int prec, bool periodic, const BatchParams& p, int block, hipStream_t s);
+hipError_t launch_batched(int prec, bool periodic, int tile_class, const BatchParams& p, int block, hipStream_t s);
 hipError_t launch_field_eval(const FieldDev& f, int prec, const double* pts, int64_t m, double rf, double* out,
                              hipStream_t s);
 hipError_t launch_gather_probe(const float* table, uint32_t bytes, uint32_t ncell, int mode, int iters,

@@ -26,7 +26,7 @@ import time
 import numpy as np
 
 from . import _lib
-from .batch import BATCH_ALGORITHMS, make_map
+from .batch import BATCH_ALGORITHMS, class_order, make_map
 from .device import DeviceArray, Event, PinnedArray, copy_d2h_async, copy_h2d_async, mem_info, synchronize
 from .engine import DEFAULT_REFINE_BITS, STAGGERED, U_STATE, ParticleState, motion_axis
 from .interpolators import DeviceFieldMap
@@ -107,10 +107,15 @@ class _FreeList:
 
 
 class _PinnedPool:
-    """Reusable page-locked host buffers (admission uploads, result downloads)."""
+    """Reusable page-locked host buffers (admission uploads, result downloads), in power-of-two sizes
+    from 1 MiB so that buffers are reused across jobs of similar sizes (page-locking is slow)."""
 
-    def __init__(self):
-        self.free = []
+    def __init__(self, prewarm=()):
+        self.free = [PinnedArray(self._size(b)) for b in prewarm]
+
+    @staticmethod
+    def _size(nbytes):
+        return 1 << max(20, (max(int(nbytes), 1) - 1).bit_length())
 
     def take(self, nbytes):
         best = None
@@ -119,7 +124,7 @@ class _PinnedPool:
                 best = k
         if best is not None:
             return self.free.pop(best)
-        return PinnedArray(max(nbytes, 1 << 20))
+        return PinnedArray(self._size(nbytes))
 
     def give(self, buf):
         self.free.append(buf)
@@ -167,7 +172,7 @@ class TrackingServer:
     """
 
     def __init__(self, maps, precision="fp32_mixed", chunk_steps=256, pool_particles=None, hbm_fraction=0.5,
-                 block_size=256, sort_every_chunks=4, max_bypass=8, max_inflight=2):
+                 block_size=256, sort_every_chunks=4, max_bypass=8, max_inflight=2, prewarm_particles=(2 ** 21,) * 4):
         if precision not in ("fp32_mixed", "fp64_fast"):
             raise ValueError("serving runs in 'fp32_mixed' or 'fp64_fast'")
         self.maps = [m if isinstance(m, DeviceFieldMap) else make_map(*m, precision=precision) for m in maps]
@@ -197,7 +202,7 @@ class TrackingServer:
         self._ws_bytes = 0
         self._stage = None          # device AoS staging of admissions (2n x 3 doubles: r then v)
         self._out = None            # device AoS staging of retirements
-        self._pinned = _PinnedPool()
+        self._pinned = _PinnedPool([48 * n for n in prewarm_particles])   # r and v of jobs that large
         self._uploads = []          # (event, pinned buffer) of admission uploads in flight
         self._slots = [_RoundSlot() for _ in range(max(1, int(max_inflight)))]
 
@@ -388,6 +393,10 @@ class TrackingServer:
         tb, ts = slot.tile_views(ntiles)
         _lib.call("pp_batch_tiles", slot.table_pin.ptr, nb, self.block_size, tb.ctypes.data, ts.ctypes.data,
                   ctypes.byref(nt))
+        # RK4 jobs' tiles first, each class to the kernel built for it (pp_tile_class)
+        tb_o, ts_o, runs = class_order(tb[:nt.value], ts[:nt.value], order)
+        tb[:nt.value] = tb_o
+        ts[:nt.value] = ts_o
         copy_h2d_async(slot.table_dev.ptr, slot.table_pin, ctypes.sizeof(_lib.PPBunch) * nb)
         copy_h2d_async(slot.tb_dev.ptr, slot.tiles_pin, 4 * ntiles)
         copy_h2d_async(slot.ts_dev.ptr, slot.tiles_pin, 8 * ntiles, offset_bytes=4 * slot.tiles_cap)
@@ -398,11 +407,13 @@ class TrackingServer:
         a.n = self.pool_n
         a.x, a.y, a.z, a.vx, a.vy, a.vz = p.x.ptr, p.y.ptr, p.z.ptr, p.vx.ptr, p.vy.ptr, p.vz.ptr
         a.bunches, a.nbunch, a.nmaps = slot.table_dev.ptr, nb, len(self.maps)
-        a.tile_bunch, a.tile_start, a.ntiles = slot.tb_dev.ptr, slot.ts_dev.ptr, nt.value
         a.step0, a.nsteps = 0, self.chunk_steps
         for k, m in enumerate(self.maps):
             a.maps[k] = m.handle
-        _lib.call("pp_push_batched", ctypes.byref(a), None)
+        for cls, first, count in runs:
+            a.tile_bunch, a.tile_start = slot.tb_dev.view_ptr(first), slot.ts_dev.view_ptr(first)
+            a.ntiles, a.tile_class = count, cls
+            _lib.call("pp_push_batched", ctypes.byref(a), None)
         self.rounds += 1
         self.particle_steps += work
         still = []

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layout_matches_binding():
     assert _lib.layout_ok()
-    assert _lib.lib().pp_abi_version() == 2
+    assert _lib.lib().pp_abi_version() == 3
 
 
 def test_constants_match_reference_expressions():
