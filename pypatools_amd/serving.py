@@ -19,7 +19,9 @@ retirement, so a job's result does not depend on which other jobs shared its
 chunks (tests/test_gpu_batch_serve.py checks alone vs in the mix, bit for bit).
 """
 import collections
+import concurrent.futures
 import ctypes
+import os
 import threading
 import time
 
@@ -130,6 +132,34 @@ class _PinnedPool:
         self.free.append(buf)
 
 
+class _HostCopier:
+    """Host copies between job arrays and page-locked buffers, split over a few threads when large
+    (NumPy releases the GIL while it copies, so the slices run in parallel)."""
+
+    SPLIT_BYTES = 8 << 20
+
+    def __init__(self, threads):
+        self.threads = max(1, int(threads))
+        self._pool = None
+
+    def __call__(self, dst, src):
+        if self.threads == 1 or dst.nbytes < self.SPLIT_BYTES:
+            np.copyto(dst, src)
+            return
+        if self._pool is None:
+            self._pool = concurrent.futures.ThreadPoolExecutor(self.threads, thread_name_prefix="pp-copy")
+        k = min(self.threads, -(-dst.nbytes // self.SPLIT_BYTES))
+        edges = np.linspace(0, len(dst), k + 1).astype(np.int64)
+        futs = [self._pool.submit(np.copyto, dst[a:b], src[a:b]) for a, b in zip(edges[:-1], edges[1:])]
+        for f in futs:
+            f.result()
+
+    def close(self):
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
+
+
 class _RoundSlot:
     """Device and pinned tile tables of one round in flight, reusable once its event has completed."""
 
@@ -169,10 +199,18 @@ class TrackingServer:
     when an event recorded after their copies has passed.  Pool slots are
     freed at enqueue time: later work on the same stream runs after the
     retirement copies.
+
+    A round admits jobs until their host data reaches ``admit_bytes_per_round``
+    (at least one job when nothing is running), so a burst of arrivals is
+    ingested over several rounds while the device already pushes the first
+    ones, instead of holding the first launch back until every upload is staged.
+    Host copies of large jobs into and out of page-locked memory are split over
+    ``copy_threads`` threads.
     """
 
     def __init__(self, maps, precision="fp32_mixed", chunk_steps=256, pool_particles=None, hbm_fraction=0.5,
-                 block_size=256, sort_every_chunks=4, max_bypass=8, max_inflight=2, prewarm_particles=(2 ** 21,) * 4):
+                 block_size=256, sort_every_chunks=4, max_bypass=8, max_inflight=2, prewarm_particles=(2 ** 21,) * 4,
+                 admit_bytes_per_round=64 << 20, copy_threads=None):
         if precision not in ("fp32_mixed", "fp64_fast"):
             raise ValueError("serving runs in 'fp32_mixed' or 'fp64_fast'")
         self.maps = [m if isinstance(m, DeviceFieldMap) else make_map(*m, precision=precision) for m in maps]
@@ -203,6 +241,8 @@ class TrackingServer:
         self._stage = None          # device AoS staging of admissions (2n x 3 doubles: r then v)
         self._out = None            # device AoS staging of retirements
         self._pinned = _PinnedPool([48 * n for n in prewarm_particles])   # r and v of jobs that large
+        self.admit_bytes_per_round = int(admit_bytes_per_round)
+        self._copy = _HostCopier(copy_threads if copy_threads is not None else min(8, os.cpu_count() or 1))
         self._uploads = []          # (event, pinned buffer) of admission uploads in flight
         self._slots = [_RoundSlot() for _ in range(max(1, int(max_inflight)))]
 
@@ -269,8 +309,8 @@ class TrackingServer:
         nb = 48 * n
         pin = self._pinned.take(nb)
         h = pin.view(np.float64, 6 * n)
-        h[:3 * n] = job.r0.reshape(-1)
-        h[3 * n:] = job.v0.reshape(-1)
+        self._copy(h[:3 * n], job.r0.reshape(-1))
+        self._copy(h[3 * n:], job.v0.reshape(-1))
         stage = self._grow("_stage", nb)
         copy_h2d_async(stage.ptr, pin, nb)
         _lib.call("pp_aos_to_soa", stage.ptr, n, p.x.view_ptr(o), p.y.view_ptr(o), p.z.view_ptr(o), None)
@@ -315,7 +355,10 @@ class TrackingServer:
         for job, ev, pin in self.retiring:
             if ev.query():
                 h = pin.view(np.float64, 6 * job.n)
-                job.result = (h[:3 * job.n].reshape(job.n, 3).copy(), h[3 * job.n:].reshape(job.n, 3).copy())
+                r, v = np.empty((job.n, 3)), np.empty((job.n, 3))
+                self._copy(r.reshape(-1), h[:3 * job.n])
+                self._copy(v.reshape(-1), h[3 * job.n:])
+                job.result = (r, v)
                 job.t_done = time.perf_counter()
                 job.state = "done"
                 self._pinned.give(pin)
@@ -338,8 +381,11 @@ class TrackingServer:
         with self.lock:
             pending = list(self.queue)
         bypassed = 0
+        staged = 0
         for job in pending:
             if job.arrival is not None and now is not None and job.arrival > now:
+                break
+            if staged >= self.admit_bytes_per_round and (admitted or self.running):
                 break
             off = self.free.alloc(job.n)
             if off is None:
@@ -352,6 +398,7 @@ class TrackingServer:
             self._admit(job, off)
             self.running.append(job)
             admitted += 1
+            staged += 48 * job.n
         return admitted
 
     def round(self, now=None):
