@@ -436,32 +436,36 @@ DEV double cevald(double2 ab, double2 cd, double2 ef, double2 gh, double tx, dou
     return fma(tz, p4, r);
 }
 
-// cell index (an integer-valued double) and weight on one non-periodic axis, u-space bounds test
+// cell index (an integer-valued double) and weight on one non-periodic axis, u-space bounds test.
+// Only points inside (ok) use the index, and there floor(u) >= 0: it is clamped at the top alone
+// (u = nc lies in the last cell).
 DEV double hot_axis(double x, double id, double c, int nc, double& t, bool& ok) {
     const double u = fma(x, id, c);
     ok = ok && (u >= 0.0) && (u <= (double)nc);
-    const double fu = fmin(fmax(floor(u), 0.0), (double)(nc - 1));
+    const double fu = fmin(floor(u), (double)(nc - 1));
     t = fmin(fmax(u - fu, 0.0), 1.0);
     return fu;
 }
 
-// periodic axis: k = floor(u) mod nc in integer arithmetic.  floor(u) is clamped
-// to +-2^29 (NaN goes to -2^29) and offset by bias = nc * floor(2^30 / nc)
-// (>= 2^29 for nc <= 2^29) into [0, 2^31); q = mulhi(ku, floor((2^32-1)/nc)) is
-// the quotient or one less, so one conditional subtraction leaves k in [0, nc)
-// for every input, and k = floor(u) mod nc whenever |floor(u)| <= 2^29.
-// A power-of-two nc (mask = nc - 1, wave-uniform) wraps with one AND, exact for every clamped floor(u).
-DEV uint32_t hot_axis_periodic(double z, double id, double c, uint32_t nc, uint32_t magic, uint32_t bias,
-                               uint32_t mask, double& t) {
-    const double u = fma(z, id, c);
+// periodic axis: k = floor(u) mod nc, as an integer-valued double.
+// A power-of-two nc (wave-uniform test) wraps in FP64: fu / nc = fu * 2^-b is exact, and so are its
+// floor and fu - nc floor(fu / nc), which lies in [0, nc) for every |fu| < 2^53; a NaN position
+// maps to cell 0 (its weight is NaN).
+// Other nc wrap in integer arithmetic: floor(u) is clamped to +-2^29 (NaN goes to -2^29) and offset
+// by bias = nc * floor(2^30 / nc) (>= 2^29 for nc <= 2^29) into [0, 2^31); q = mulhi(ku,
+// floor((2^32-1)/nc)) is the quotient or one less, so one conditional subtraction leaves k in
+// [0, nc) for every input, and k = floor(u) mod nc whenever |floor(u)| <= 2^29.
+DEV double hot_axis_periodic(double z, const HotMap& m, double& t) {
+    const double u = fma(z, m.id2, m.c2);
     const double fu = floor(u);
     t = u - fu;
+    if (m.zmask != 0u) return fmax(fma(-m.nc2d, floor(fu * m.zinv), fu), 0.0);
+    const uint32_t nc = (uint32_t)m.nc2;
     const int32_t ki = (int32_t)fmin(fmax(fu, -536870912.0), 536870912.0);
-    if (mask != 0u) return (uint32_t)ki & mask;
-    const uint32_t ku = (uint32_t)ki + bias;
-    uint32_t k = ku - __umulhi(ku, magic) * nc;
+    const uint32_t ku = (uint32_t)ki + m.zbias;
+    uint32_t k = ku - __umulhi(ku, m.zmagic) * nc;
     if (k >= nc) k -= nc;
-    return k;
+    return (double)k;
 }
 
 // The hot paths take coefficient tables below 2^31 bytes (192 or 384 per cell), so every cell index is
@@ -475,8 +479,7 @@ DEV uint32_t hot_locate(const HotMap& m, double x, double y, double z, double& t
     const double i = hot_axis(x, m.id0, m.c0, m.nc0, tx, ok);
     const double j = hot_axis(y, m.id1, m.c1, m.nc1, ty, ok);
     double k;
-    if constexpr (PERIODIC)
-        k = (double)hot_axis_periodic(z, m.id2, m.c2, (uint32_t)m.nc2, m.zmagic, m.zbias, m.zmask, tz);
+    if constexpr (PERIODIC) k = hot_axis_periodic(z, m, tz);
     else k = hot_axis(z, m.id2, m.c2, m.nc2, tz, ok);
     return (uint32_t)fma(i, m.nc12d, fma(j, m.nc2d, k));
 }

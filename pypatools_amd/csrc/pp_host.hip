@@ -454,6 +454,7 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
         h.zmagic = (uint32_t)(0xFFFFFFFFull / (uint64_t)m.nc[2]);
         h.zbias = (uint32_t)m.nc[2] * ((1u << 30) / (uint32_t)m.nc[2]);
         if ((m.nc[2] & (m.nc[2] - 1)) == 0) h.zmask = (uint32_t)m.nc[2] - 1u;
+        h.zinv = 1.0 / (double)m.nc[2];
     }
     h.sE = 1.0;
     h.sB = 1.0;

@@ -80,7 +80,8 @@ struct HotMap {
     uint32_t ncells;        // cells in the table (nc0 nc1 nc2)
     uint32_t zmagic;        // floor((2^32 - 1) / nc2): quotient estimate for the periodic z wrap
     uint32_t zbias;         // nc2 * floor(2^30 / nc2): makes the z cell index non-negative before the wrap
-    uint32_t zmask;         // nc2 - 1 when nc2 is a power of two (the wrap is then one AND), else 0
+    uint32_t zmask;         // nc2 - 1 when nc2 is a power of two, else 0
+    double zinv;            // 1 / nc2 (exact when nc2 is a power of two)
 };
 
 // RF factors of the hot kernels: table [step - step0][stage][cos, sin]; e, b = pp_rf_mode of E and B
