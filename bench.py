@@ -33,18 +33,11 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TFLOPS = 78.6       # MI355X vector peak without packed FP32 (= FP64 vector): 256 CUs x 128 flop/clk x 2.4 GHz
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4        # wave64 VALU instructions per second, whole chip
 # sources of the hot kernel; counters in profiles/pmc_counters.json are used only for this exact source
-KERNEL_SOURCES = ("pypatools_amd/csrc/pp_fast.hip", "pypatools_amd/csrc/pp_internal.h",
-                  "pypatools_amd/csrc/pp_fp64.inc", "pypatools_amd/csrc/pp_bvh.inc")
-
-
 def kernel_source_digest():
-    """sha256 over the hot kernel's source files (the build's inputs for pp_fast.hip)."""
-    import hashlib
-    h = hashlib.sha256()
-    for rel in KERNEL_SOURCES:
-        with open(os.path.join(ROOT, rel), "rb") as f:
-            h.update(f.read())
-    return h.hexdigest()
+    """sha256 of the fast-path kernel sources the loaded library was built from (compiled into it by
+    pypatools_amd/build.py); counters are matched to the binary that runs, not to the files on disk."""
+    from pypatools_amd import _lib
+    return _lib.kernel_digest()
 
 
 def load_counters(precision):
@@ -64,7 +57,7 @@ def load_counters(precision):
     return rec, "counters of this source"
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -101,7 +94,7 @@ def parse():
     ap.add_argument("--pmc-steps-per-launch", type=int, default=500)
     ap.add_argument("--write-counters", action="store_true",
                     help="with --summarize-pmc: store the record in profiles/pmc_counters.json")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     if args.refine_bits is None:
         from pypatools_amd.engine import DEFAULT_REFINE_BITS
         args.refine_bits = DEFAULT_REFINE_BITS
@@ -166,6 +159,12 @@ def summarize_pmc(args):
         out["per_wavefront_step"] = {name: k[name] / (waves * args.pmc_steps_per_launch)
                                      for name in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD",
                                                   "SQ_INSTS_SMEM") if name in k}
+    dur_ns = out.get("kernel", {}).get("mean_duration_ns")
+    if "GRBM_GUI_ACTIVE" in k and dur_ns:
+        # GRBM_GUI_ACTIVE sums the busy cycles of every XCD: the shader clock the kernel ran at
+        xcc = num_xcc(args.pmc_dirs)
+        out["clock"] = {"GHz": k["GRBM_GUI_ACTIVE"] / xcc / dur_ns, "xcc": xcc,
+                        "note": "GRBM_GUI_ACTIVE / XCDs / dispatch duration, under the profiler"}
     print(json.dumps(out, indent=1), flush=True)
     if args.write_counters:
         cfile = os.path.join(ROOT, "profiles", "pmc_counters.json")
@@ -183,6 +182,8 @@ def summarize_pmc(args):
         if "hbm" in out:
             rec["hbm_bytes_per_launch"] = out["hbm"]["bytes_per_dispatch"]
             rec["hbm_bytes_per_particle_step"] = out["hbm"]["bytes_per_dispatch"] / units
+        if "clock" in out:
+            rec["clock_GHz"] = out["clock"]["GHz"]
         for name, key in (("SQ_INSTS_VALU", "valu_insts_per_wavefront_step"),
                           ("SQ_INSTS_LDS", "lds_insts_per_wavefront_step"),
                           ("SQ_INSTS_VMEM_RD", "vmem_insts_per_wavefront_step"),
@@ -192,6 +193,19 @@ def summarize_pmc(args):
         data[args.precision] = rec
         with open(cfile, "w") as f:
             json.dump(data, f, indent=1, sort_keys=True)
+
+
+def num_xcc(dirs):
+    """XCD count of the profiled GPU (rocprofv3 agent info), 8 on MI355X when absent."""
+    import csv
+    import glob
+    for d in dirs:
+        for fn in glob.glob(os.path.join(d, "**", "*agent_info.csv"), recursive=True):
+            with open(fn, newline="") as f:
+                for row in csv.DictReader(f):
+                    if row.get("Agent_Type") == "GPU" and row.get("Num_Xcc"):
+                        return int(row["Num_Xcc"])
+    return 8
 
 
 def copy_probe(mib):
@@ -589,15 +603,19 @@ def run_c5(args, n_jobs=None):
         Ek = {c: s * E[c] for c in "xyz"}
         Bk = {c: (2.0 - s) * B[c] for c in "xyz"}
         fields.append((pa.Field.from_arrays(grid, Ek, periodic_z=True), pa.Field.from_arrays(grid, Bk, periodic_z=True)))
-    srv = TrackingServer(fields, args.precision, chunk_steps=256, pool_particles=args.pool)
-    mean_work = 2.63e5 * 3.76e3       # E[N] E[steps] of the C5 distributions
+    srv = TrackingServer(fields, args.precision, chunk_steps=128, pool_particles=args.pool)
+    n_jobs = args.jobs if n_jobs is None else n_jobs
+    # the job set of the run: its contents do not depend on the arrival rate (c5_jobs draws them in the
+    # same order for any rate), so the closed batch below and the timed run push the same jobs
+    cal = c5_jobs(n_jobs, 1e9)
+    mean_work = float(np.mean([j.n * j.nsteps for j in cal]))
     capacity = args.capacity
     measured = None
     if capacity <= 0:
-        # capacity of the server on the C5 job mix (Boris / Vay / RK4, which costs 4 gathers per step):
-        # a closed batch of C5-distributed jobs submitted at once, timed until the queue has emptied and
-        # fewer than 8 jobs are left running (the drain of the last few long jobs is not capacity)
-        cal = c5_jobs(max(args.jobs // 2, 60), 1e9, seed=11)
+        # capacity of the server on this job set (Boris / Vay / RK4 jobs, RK4 costing 4 gathers per step;
+        # a few long jobs hold most of the work, so the mix of one set differs from the next): the jobs
+        # submitted at once as a closed batch, timed until the queue has emptied and fewer than 8 jobs are
+        # left running (the drain of the last few long jobs is not capacity)
         warm = c5_jobs(4, 1e9, seed=12)          # first admissions allocate the page-locked buffers
         for j in warm:
             srv.submit(j)
@@ -618,7 +636,7 @@ def run_c5(args, n_jobs=None):
         srv.particle_steps = 0
         srv.rounds = 0
     lam = args.load * capacity / mean_work
-    jobs = c5_jobs(args.jobs if n_jobs is None else n_jobs, lam)
+    jobs = c5_jobs(n_jobs, lam)
     t0 = time.perf_counter()
     for j in jobs:
         j.arrival += t0
@@ -657,7 +675,7 @@ def bench_c5(args):
                                   f"({args.load:.0%} of {capacity:.2e} particle-steps/s "
                                   f"{'measured ' if measured else ''}capacity), N log-uniform "
                                   "1e3-2e6, steps log-uniform 1e2-2e4, Boris/Vay/RK4, 4 resident 128^3 maps, "
-                                  "256-step chunks", "precision": args.precision, "pool_particles": srv.pool_n},
+                                  "128-step chunks", "precision": args.precision, "pool_particles": srv.pool_n},
            "extra": summ}
     print(json.dumps(out), flush=True)
 
@@ -840,6 +858,10 @@ def main():
                       "frac": rec["valu_insts_per_wavefront_step"] * wave_steps / slots,
                       "slots": "256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz",
                       "flop_efficiency_per_instruction": FLOPS_PER_STEP / (2 * rec["valu_insts_per_wavefront_step"])})
+        if rec.get("clock_GHz"):
+            # the same at the shader clock the kernel ran at under the profiler (below the 2.4 GHz peak)
+            issue["clock_GHz_measured"] = rec["clock_GHz"]
+            issue["frac_at_measured_clock"] = issue["frac"] * 2.4 / rec["clock_GHz"]
     out = {
         "metric": "particle-steps/sec on 1 MI355X, FP32-mixed vs FP64 at stated error; p99 job latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

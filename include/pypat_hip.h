@@ -137,6 +137,9 @@ typedef struct pp_push_args {
 int pp_init(int device);                                   /* select device, create default stream */
 const char* pp_last_error(void);
 int pp_abi_version(void);
+/* sha256 (hex) of the fast-path kernel sources the library was built from (no counterpart in the
+   reference): profiler counters recorded with it are matched to the binary that runs. */
+const char* pp_kernel_digest(void);
 /* The constants the kernels use, as the reference's Python expressions give them:
    out[0] CLIGHT, out[1] CLIGHT**2 (pusher.py:90), out[2] 0.9999*CLIGHT (:276),
    out[3..5] YOSHIDA_C1..C3, out[6..8] YOSHIDA_D1..D3 (:176-181).  n >= 9. */

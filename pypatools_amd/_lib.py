@@ -95,6 +95,7 @@ SIGNATURES = {
     "pp_init": (c_int32, [ctypes.c_int]),
     "pp_last_error": (ctypes.c_char_p, []),
     "pp_abi_version": (c_int32, []),
+    "pp_kernel_digest": (ctypes.c_char_p, []),
     "pp_get_constants": (c_int32, [c_double_p, c_int32]),
     "pp_struct_sizes": (c_int32, [ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "pp_batch_struct_sizes": (c_int32, [ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
@@ -231,6 +232,11 @@ def check(rc):
 
 def call(name, *args):
     check(getattr(lib(), name)(*args))
+
+
+def kernel_digest():
+    """sha256 of the fast-path kernel sources the loaded library was built from."""
+    return lib().pp_kernel_digest().decode()
 
 
 def layout_ok():

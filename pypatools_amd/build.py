@@ -28,6 +28,21 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+# the fast-path kernel sources: their sha256 is compiled into the library (pp_kernel_digest), so that
+# profiler counters taken on one build are matched to the binary that runs, not to the files on disk
+KERNEL_SOURCES = ("pp_fast.hip", "pp_internal.h", "pp_fp64.inc", "pp_bvh.inc")
+
+
+def kernel_source_digest():
+    """sha256 over the fast-path kernel sources as they are on disk."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(CSRC, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 PER_FILE = {
     "pp_strict.hip": ["-ffp-contract=off"],
     "pp_collide.hip": ["-ffp-contract=off"],
@@ -43,9 +58,10 @@ def hipcc():
     return exe
 
 
-def _compile(src):
+def _compile(src, digest):
     obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
-    cmd = [hipcc(), "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + PER_FILE.get(src, [])
+    extra = [f'-DPP_KERNEL_DIGEST="{digest}"'] if src == "pp_host.hip" else []
+    cmd = [hipcc(), "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + PER_FILE.get(src, []) + extra
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr[-6000:]}")
@@ -54,8 +70,9 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(BUILD_DIR, exist_ok=True)
+    digest = kernel_source_digest()
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, digest), SOURCES))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     res = subprocess.run(cmd, capture_output=True, text=True)

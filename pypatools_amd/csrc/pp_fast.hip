@@ -129,7 +129,7 @@ struct GenericFE32 {
     }
 };
 
-// Field evaluators with raw_e() (kRawE) hand E over before its multiplier.
+// Field evaluators with raw() (kRawE) hand E and B over before their multipliers.
 template <class FE, class = void>
 struct HasRawE {
     static constexpr bool value = false;
@@ -175,13 +175,16 @@ DEV void step_mixed(int64_t i, d3& r, d3& v, d3& vrec, double dt, float h, int64
         if constexpr (REL) v = clamp_speed(v, p.vclamp);
         vrec = v;
     } else if constexpr (ALG == PP_ALG_BORIS && PRE && HasRawE<FE>::value) {
-        // as below, with E's multiplier kE (h times its scale) folded into two FMAs per component
-        float kE;
-        fe.raw_e(i, r.x, r.y, r.z, gs, 0, E, B, kE);
+        // as below, with the field multipliers (h times the scales) folded in: E's kE into two FMAs per
+        // component; B's kB into the rotation, t = kB b:  s = 2 t / (1 + t^2) = (2 kB / (1 + kB^2 b^2)) b
+        // and v- x t = kB (v- x b), so t itself is never formed
+        float kE, kB;
+        fe.raw(i, r.x, r.y, r.z, gs, 0, E, B, kE, kB);
         const f3 vf = tof(v);
         const f3 vmf = mkf(fmaf(kE, E.x, vf.x), fmaf(kE, E.y, vf.y), fmaf(kE, E.z, vf.z));
-        const f3 s = (2.f * rcpf(1.f + norm2f(B))) * B;
-        const f3 vp = vmf + crossf(vmf, B);
+        const f3 s = ((2.f * kB) * rcpf(fmaf(kB * kB, norm2f(B), 1.f))) * B;
+        const f3 cb = crossf(vmf, B);
+        const f3 vp = mkf(fmaf(kB, cb.x, vmf.x), fmaf(kB, cb.y, vmf.y), fmaf(kB, cb.z, vmf.z));
         const f3 c = crossf(vp, s);
         const float k2 = 2.f * kE;
         v = addf(v, mkf(fmaf(k2, E.x, c.x), fmaf(k2, E.y, c.y), fmaf(k2, E.z, c.z)));
@@ -647,7 +650,7 @@ struct HotFE32 {
     mutable uint32_t last_cell = 0;   // REUSE: this lane's cell and LDS slot at the last staging
     mutable uint32_t last_slot = 0;
     mutable bool staged = false;      // REUSE: the wavefront has staged at least once (wave-uniform)
-    static constexpr bool kRawE = true;   // raw_e(): E before its multiplier (step_mixed<BORIS> folds it)
+    static constexpr bool kRawE = true;   // raw(): E and B before their multipliers (step_mixed<BORIS> folds them)
 
     // field multipliers at global step gs and RK stage (RF factors applied)
     DEV void scales(int64_t gs, int stage, float& kE, float& kB) const {
@@ -667,13 +670,14 @@ struct HotFE32 {
         B = mkf(kB * v[3], kB * v[4], kB * v[5]);
     }
 
-    // E unscaled with its multiplier kE, B scaled
-    DEV void raw_e(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B, float& kE) const {
-        float v[6], kB;
+    // E and B unscaled, with their multipliers kE and kB
+    DEV void raw(int64_t, double x, double y, double z, int64_t gs, int stage, f3& E, f3& B, float& kE,
+                 float& kB) const {
+        float v[6];
         gather(x, y, z, v);
         scales(gs, stage, kE, kB);
         E = mkf(v[0], v[1], v[2]);
-        B = mkf(kB * v[3], kB * v[4], kB * v[5]);
+        B = mkf(v[3], v[4], v[5]);
     }
 
     // the six map components (E then B) at a point, unscaled

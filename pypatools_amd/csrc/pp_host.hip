@@ -57,6 +57,11 @@ extern "C" {
 
 int pp_abi_version(void) { return PP_ABI_VERSION; }
 
+#ifndef PP_KERNEL_DIGEST
+#define PP_KERNEL_DIGEST "unknown"
+#endif
+const char* pp_kernel_digest(void) { return PP_KERNEL_DIGEST; }
+
 int pp_struct_sizes(int64_t* field_size, int64_t* push_args_size) {
     if (!field_size || !push_args_size) return fail(PP_ERR_INVALID, "null argument");
     *field_size = (int64_t)sizeof(pp_field);

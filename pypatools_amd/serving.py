@@ -8,9 +8,10 @@ and a free list of slot ranges.  Each scheduling round it
      yet waits while smaller ones behind it may go ahead, at most
      ``max_bypass`` times) -- admission is by free HBM: the pool is the HBM
      budget for particle state;
-  2. advances every running job by one chunk (``chunk_steps``, 256 by
-     default) in ONE batched launch, so the work of all running jobs overlaps
-     on the device while their state stays resident between chunks;
+  2. advances every running job by one chunk (``chunk_steps``, 128 by
+     default) in one batched launch per tile class (RK4 jobs, the others), so
+     the work of all running jobs overlaps on the device while their state
+     stays resident between chunks;
   3. retires finished jobs: results to the host, slot range freed.
 
 Each job's particles are cell-sorted inside its own range at admission (and
@@ -22,7 +23,6 @@ import collections
 import concurrent.futures
 import ctypes
 import os
-import queue
 import threading
 import time
 
@@ -66,9 +66,6 @@ class Job:
         self.t_submit = self.t_admit = self.t_done = None
         self.result = None
         self.event = threading.Event()
-        self._pin = None                      # page-locked copy of r0 and v0 (staged ahead of admission)
-        self._uploaded = False                # the copy has gone to the device (admission took it)
-        self._stage_lock = threading.Lock()
 
     @property
     def latency(self):
@@ -118,25 +115,22 @@ class _PinnedPool:
 
     def __init__(self, prewarm=()):
         self.free = [PinnedArray(self._size(b)) for b in prewarm]
-        self._lock = threading.Lock()
 
     @staticmethod
     def _size(nbytes):
         return 1 << max(20, (max(int(nbytes), 1) - 1).bit_length())
 
     def take(self, nbytes):
-        with self._lock:
-            best = None
-            for k, b in enumerate(self.free):
-                if b.nbytes >= nbytes and (best is None or b.nbytes < self.free[best].nbytes):
-                    best = k
-            if best is not None:
-                return self.free.pop(best)
+        best = None
+        for k, b in enumerate(self.free):
+            if b.nbytes >= nbytes and (best is None or b.nbytes < self.free[best].nbytes):
+                best = k
+        if best is not None:
+            return self.free.pop(best)
         return PinnedArray(self._size(nbytes))
 
     def give(self, buf):
-        with self._lock:
-            self.free.append(buf)
+        self.free.append(buf)
 
 
 class _HostCopier:
@@ -148,15 +142,13 @@ class _HostCopier:
     def __init__(self, threads):
         self.threads = max(1, int(threads))
         self._pool = None
-        self._lock = threading.Lock()
 
     def __call__(self, dst, src):
         if self.threads == 1 or dst.nbytes < self.SPLIT_BYTES:
             np.copyto(dst, src)
             return
-        with self._lock:
-            if self._pool is None:
-                self._pool = concurrent.futures.ThreadPoolExecutor(self.threads, thread_name_prefix="pp-copy")
+        if self._pool is None:
+            self._pool = concurrent.futures.ThreadPoolExecutor(self.threads, thread_name_prefix="pp-copy")
         k = min(self.threads, -(-dst.nbytes // self.SPLIT_BYTES))
         edges = np.linspace(0, len(dst), k + 1).astype(np.int64)
         futs = [self._pool.submit(np.copyto, dst[a:b], src[a:b]) for a, b in zip(edges[:-1], edges[1:])]
@@ -167,38 +159,6 @@ class _HostCopier:
         if self._pool is not None:
             self._pool.shutdown(wait=True)
             self._pool = None
-
-
-def _stage_job(job, pinned, copy, take=False):
-    """The job's r0 and v0 in a page-locked buffer, copied once, by the staging thread or by admission,
-    whichever comes first.  take (admission): hand the buffer over to the upload.
-    Returns (buffer, True if this call made the copy)."""
-    with job._stage_lock:
-        if job._uploaded:
-            return None, False
-        copied = job._pin is None
-        if copied:
-            n = job.n
-            pin = pinned.take(48 * n)
-            h = pin.view(np.float64, 6 * n)
-            copy(h[:3 * n], job.r0.reshape(-1))
-            copy(h[3 * n:], job.v0.reshape(-1))
-            job._pin = pin
-        pin = job._pin
-        if take:
-            job._pin = None
-            job._uploaded = True
-        return pin, copied
-
-
-def _stage_worker(q, pinned, copy):
-    """Staging thread: copies submitted jobs into page-locked buffers until it takes None.  It holds the
-    queue, the pool and the copier, not the server, so a dropped server is collected (and ends it)."""
-    while True:
-        job = q.get()
-        if job is None:
-            return
-        _stage_job(job, pinned, copy)
 
 
 class _RoundSlot:
@@ -241,17 +201,17 @@ class TrackingServer:
     freed at enqueue time: later work on the same stream runs after the
     retirement copies.
 
-    Submitted jobs are copied into page-locked buffers by a staging thread, ahead
-    of their admission, so that a round only enqueues their uploads.  A job
-    that is admitted before the thread has reached it is copied by the round
-    itself, and a round copies at most ``admit_bytes_per_round`` of host data
-    that way (at least one job when nothing is running): a burst of arrivals is
-    then ingested while the device already pushes the first jobs.  Host copies
-    of large jobs into and out of page-locked memory are split over
-    ``copy_threads`` threads.  ``close()`` ends the staging thread.
+    A round admits jobs until their host data reaches ``admit_bytes_per_round``
+    (at least one job when nothing is running), so a burst of arrivals is
+    ingested over several rounds while the device already pushes the first
+    ones, instead of holding the first launch back until every upload is staged.
+    Host copies of large jobs into and out of page-locked memory are split over
+    ``copy_threads`` threads.  (Copying every submitted job ahead of admission on
+    a thread of its own measured slower: it page-locks a buffer per queued job
+    while the round loop launches.)
     """
 
-    def __init__(self, maps, precision="fp32_mixed", chunk_steps=256, pool_particles=None, hbm_fraction=0.5,
+    def __init__(self, maps, precision="fp32_mixed", chunk_steps=128, pool_particles=None, hbm_fraction=0.5,
                  block_size=256, sort_every_chunks=4, max_bypass=8, max_inflight=2, prewarm_particles=(2 ** 20,),
                  admit_bytes_per_round=64 << 20, copy_threads=None):
         if precision not in ("fp32_mixed", "fp64_fast"):
@@ -286,10 +246,6 @@ class TrackingServer:
         self._pinned = _PinnedPool([48 * n for n in prewarm_particles])   # r and v of jobs that large
         self.admit_bytes_per_round = int(admit_bytes_per_round)
         self._copy = _HostCopier(copy_threads if copy_threads is not None else min(8, os.cpu_count() or 1))
-        self._stage_q = queue.Queue()
-        self._stager = threading.Thread(target=_stage_worker, args=(self._stage_q, self._pinned, self._copy),
-                                        name="pp-stage", daemon=True)
-        self._stager.start()
         self._uploads = []          # (event, pinned buffer) of admission uploads in flight
         self._slots = [_RoundSlot() for _ in range(max(1, int(max_inflight)))]
 
@@ -305,19 +261,7 @@ class TrackingServer:
         job.state = "queued"
         with self.lock:
             self.queue.append(job)
-        self._stage_q.put(job)
         return job
-
-    def close(self):
-        """End the staging thread (jobs still queued are staged by their admission)."""
-        if self._stager.is_alive():
-            self._stage_q.put(None)
-            self._stager.join()
-
-    def __del__(self):
-        q = getattr(self, "_stage_q", None)
-        if q is not None:
-            q.put(None)
 
     @property
     def idle(self):
@@ -364,12 +308,14 @@ class TrackingServer:
                   self.pool.ids.view_ptr(job.offset), ws.ptr, ws.nbytes, None)
 
     def _admit(self, job, offset):
-        """Enqueue the job's upload into its pool range (r, v through page-locked memory), ids, u and sort.
-        Returns the bytes it copied on the host (0 when the staging thread had copied them)."""
+        """Enqueue the job's upload into its pool range (r, v through page-locked memory), ids, u and sort."""
         job.offset = offset
         p, o, n = self.pool, offset, job.n
         nb = 48 * n
-        pin, copied = _stage_job(job, self._pinned, self._copy, take=True)   # released after the upload (_poll)
+        pin = self._pinned.take(nb)
+        h = pin.view(np.float64, 6 * n)
+        self._copy(h[:3 * n], job.r0.reshape(-1))
+        self._copy(h[3 * n:], job.v0.reshape(-1))
         stage = self._grow("_stage", nb)
         copy_h2d_async(stage.ptr, pin, nb)
         _lib.call("pp_aos_to_soa", stage.ptr, n, p.x.view_ptr(o), p.y.view_ptr(o), p.z.view_ptr(o), None)
@@ -385,7 +331,6 @@ class TrackingServer:
         self._uploads.append((ev, pin))
         job.state = "running"
         job.t_admit = time.perf_counter()
-        return nb if copied else 0
 
     def _retire(self, job):
         """Enqueue the job's unsort and download; it completes when its event has passed (_poll)."""
@@ -455,9 +400,10 @@ class TrackingServer:
                 continue
             with self.lock:
                 self.queue.remove(job)
-            staged += self._admit(job, off)
+            self._admit(job, off)
             self.running.append(job)
             admitted += 1
+            staged += 48 * job.n
         return admitted
 
     def round(self, now=None):

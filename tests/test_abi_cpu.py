@@ -32,6 +32,14 @@ def test_struct_layout_matches_binding():
     assert _lib.lib().pp_abi_version() == 3
 
 
+def test_library_was_built_from_these_kernel_sources():
+    """The sha256 compiled into the library equals that of the kernel sources on disk (a stale build
+    would otherwise be timed and profiled under the sources' name)."""
+    from pypatools_amd import build
+    assert re.fullmatch(r"[0-9a-f]{64}", _lib.kernel_digest())
+    assert _lib.kernel_digest() == build.kernel_source_digest()
+
+
 def test_constants_match_reference_expressions():
     from oracle import kernels as K
     from oracle.constants import CLIGHT

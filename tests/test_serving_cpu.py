@@ -1,14 +1,12 @@
 """Host logic of the batched tracker and the tracking server (no device calls): the tile-class
 partition, the pool's free list, the page-locked buffer sizes, the threaded host copies and the
-stage-once hand-over between the staging thread and admission."""
-import threading
-
+submit-once rule."""
 import numpy as np
 import pytest
 
 from pypatools_amd import _lib
 from pypatools_amd.batch import class_order
-from pypatools_amd.serving import Job, _FreeList, _HostCopier, _PinnedPool, _stage_job
+from pypatools_amd.serving import Job, _FreeList, _HostCopier, _PinnedPool
 
 
 class _B:
@@ -79,76 +77,9 @@ def test_host_copier_copies_every_slice(threads):
     cp.close()
 
 
-class _HostBuf:
-    """Stand-in for a page-locked buffer (PinnedArray's view())."""
-
-    def __init__(self, nbytes):
-        self.nbytes = nbytes
-        self.mem = np.zeros(nbytes, dtype=np.uint8)
-
-    def view(self, dtype, count, offset_bytes=0):
-        return self.mem[offset_bytes:].view(dtype)[:count]
-
-
-class _Pool:
-    def __init__(self):
-        self.taken = 0
-        self.lock = threading.Lock()
-
-    def take(self, nbytes):
-        with self.lock:
-            self.taken += 1
-        return _HostBuf(_PinnedPool._size(nbytes))
-
-
-def _job(n, seed):
-    rng = np.random.default_rng(seed)
-    return Job(rng.normal(size=(n, 3)), rng.normal(size=(n, 3)), 1.0, "boris", 10, 1e-11)
-
-
-def test_stage_once_then_hand_over():
-    pool, cp = _Pool(), _HostCopier(2)
-    j = _job(1000, 1)
-    pin, copied = _stage_job(j, pool, cp)                  # the staging thread's copy
-    assert copied and pool.taken == 1
-    h = pin.view(np.float64, 6 * j.n)
-    np.testing.assert_array_equal(h[:3 * j.n], j.r0.reshape(-1))
-    np.testing.assert_array_equal(h[3 * j.n:], j.v0.reshape(-1))
-    pin2, copied2 = _stage_job(j, pool, cp, take=True)     # admission takes the same buffer, no second copy
-    assert pin2 is pin and not copied2 and pool.taken == 1
-    assert _stage_job(j, pool, cp) == (None, False)       # a late staging call after the upload does nothing
-    assert pool.taken == 1
-
-
-def test_admission_first_copies_inline():
-    pool, cp = _Pool(), _HostCopier(1)
-    j = _job(10, 2)
-    pin, copied = _stage_job(j, pool, cp, take=True)
-    assert copied and pool.taken == 1
-    np.testing.assert_array_equal(pin.view(np.float64, 3 * j.n), j.r0.reshape(-1))
-    assert _stage_job(j, pool, cp) == (None, False)
-
-
-def test_stager_and_admission_race_copy_once():
-    """Many jobs staged by a thread while the main thread admits them in order: each is copied exactly
-    once and every admitted buffer holds its own job's data."""
-    pool, cp = _Pool(), _HostCopier(2)
-    jobs = [_job(int(n), k) for k, n in enumerate(np.random.default_rng(9).integers(1, 40000, 60))]
-    t = threading.Thread(target=lambda: [_stage_job(j, pool, cp) for j in jobs])
-    t.start()
-    copies = 0
-    for j in jobs:
-        pin, copied = _stage_job(j, pool, cp, take=True)
-        copies += copied
-        h = pin.view(np.float64, 6 * j.n)
-        np.testing.assert_array_equal(h[:3 * j.n], j.r0.reshape(-1))
-        np.testing.assert_array_equal(h[3 * j.n:], j.v0.reshape(-1))
-    t.join()
-    assert pool.taken == len(jobs)
-
-
 def test_job_is_submitted_once():
-    j = _job(4, 3)
+    rng = np.random.default_rng(3)
+    j = Job(rng.normal(size=(4, 3)), rng.normal(size=(4, 3)), 1.0, "boris", 10, 1e-11)
     j.state = "queued"
 
     class _S:                                              # submit()'s checks only, without a device
