@@ -42,6 +42,7 @@ struct pp_mesh {
     pp_bvh::Node* nodes;
     pp_bvh::Tri* tris;
     pp_bvh::Range* ranges;  // candidate grids, all levels
+    uint8_t* clear;         // level-0 clearance (MeshDev::clear)
     pp_bvh::MeshDev* dev;   // device copy of the descriptor
     int32_t nnodes;
     int64_t ntris;
@@ -125,6 +126,41 @@ struct Candidates {
     int32_t nlevels = 0;
     int64_t off[pp_bvh::kMaxLevels] = {};
     std::vector<pp_bvh::Range> ranges;
+    std::vector<uint8_t> clear;   // level 0: Chebyshev distance to the nearest non-empty cell, capped at 255
+
+    // Multi-source breadth-first search over the 26 neighbours of each cell: a
+    // search front that advances one step per ring reaches a cell at exactly its
+    // Chebyshev distance from the nearest non-empty cell.
+    void build_clearance(const std::vector<int32_t>& level0) {
+        const int64_t total = (int64_t)n0[0] * n0[1] * n0[2];
+        clear.assign((size_t)total, 255);
+        std::vector<int64_t> front, next;
+        for (int64_t c = 0; c < total; ++c)
+            if (level0[(size_t)c] >= 0) {
+                clear[(size_t)c] = 0;
+                front.push_back(c);
+            }
+        for (int d = 1; d < 255 && !front.empty(); ++d) {
+            next.clear();
+            for (const int64_t c : front) {
+                const int32_t i = (int32_t)(c / ((int64_t)n0[1] * n0[2]));
+                const int32_t j = (int32_t)((c / n0[2]) % n0[1]);
+                const int32_t q = (int32_t)(c % n0[2]);
+                for (int32_t di = -1; di <= 1; ++di)
+                    for (int32_t dj = -1; dj <= 1; ++dj)
+                        for (int32_t dq = -1; dq <= 1; ++dq) {
+                            const int32_t a = i + di, b = j + dj, e = q + dq;
+                            if (a < 0 || b < 0 || e < 0 || a >= n0[0] || b >= n0[1] || e >= n0[2]) continue;
+                            const int64_t t = ((int64_t)a * n0[1] + b) * n0[2] + e;
+                            if (clear[(size_t)t] == 255) {
+                                clear[(size_t)t] = (uint8_t)d;
+                                next.push_back(t);
+                            }
+                        }
+            }
+            front.swap(next);
+        }
+    }
 
     int32_t cell(int a, double x) const {
         const double f = std::floor((x - lo[a]) * inv[a]);
@@ -189,6 +225,7 @@ struct Candidates {
                         c = lca(bld, c, leaf_of[k]);
                     }
         }
+        build_clearance(cur);
         int32_t n[3] = {n0[0], n0[1], n0[2]};
         ranges.clear();
         for (nlevels = 0; nlevels < pp_bvh::kMaxLevels; ++nlevels) {
@@ -268,6 +305,7 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
     m->nodes = nullptr;
     m->tris = nullptr;
     m->ranges = nullptr;
+    m->clear = nullptr;
     m->dev = nullptr;
     m->range_cells = nt > 0 ? (int64_t)cand.ranges.size() : 0;
     hipGetDevice(&m->device);
@@ -277,11 +315,13 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
         bool ok = hipMalloc(&m->nodes, sizeof(Node) * m->nnodes) == hipSuccess &&
                   hipMalloc(&m->tris, sizeof(Tri) * (size_t)nt) == hipSuccess &&
                   hipMalloc(&m->ranges, rbytes) == hipSuccess &&
+                  hipMalloc(&m->clear, cand.clear.size()) == hipSuccess &&
                   hipMalloc(&m->dev, sizeof(pp_bvh::MeshDev)) == hipSuccess;
         if (ok) {
             d.nodes = m->nodes;
             d.tris = m->tris;
             d.ranges = m->ranges;
+            d.clear = m->clear;
             d.nnodes = m->nnodes;
             d.nlevels = cand.nlevels;
             for (int a = 0; a < 3; ++a) {
@@ -294,6 +334,7 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
             ok = hipMemcpy(m->nodes, bld.nodes.data(), sizeof(Node) * m->nnodes, hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemcpy(m->tris, sorted.data(), sizeof(Tri) * (size_t)nt, hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemcpy(m->ranges, cand.ranges.data(), rbytes, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(m->clear, cand.clear.data(), cand.clear.size(), hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemcpy(m->dev, &d, sizeof(d), hipMemcpyHostToDevice) == hipSuccess;
         }
         if (!ok) {
@@ -311,6 +352,7 @@ void destroy(pp_mesh* m) {
     if (m->nodes) (void)hipFree(m->nodes);
     if (m->tris) (void)hipFree(m->tris);
     if (m->ranges) (void)hipFree(m->ranges);
+    if (m->clear) (void)hipFree(m->clear);
     if (m->dev) (void)hipFree(m->dev);
     delete m;
 }
