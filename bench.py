@@ -90,7 +90,7 @@ def parse(argv=None):
                     help="summarise rocprofv3 --pmc CSV output (--pmc-dirs, --copy-dirs) instead of running")
     ap.add_argument("--pmc-dirs", nargs="*", default=[])
     ap.add_argument("--copy-dirs", nargs="*", default=[])
-    ap.add_argument("--kernel-regex", default=r"k_hot<1, true, true, false>")
+    ap.add_argument("--kernel-regex", default=r"k_hot<1, true, true, false, false>")
     ap.add_argument("--pmc-steps-per-launch", type=int, default=500)
     ap.add_argument("--write-counters", action="store_true",
                     help="with --summarize-pmc: store the record in profiles/pmc_counters.json")
@@ -302,8 +302,8 @@ def collision_rate(st_old, st_new, reps=5):
 def lossy_tracking(w, fp, qm, nsteps=2000, radii=(3e-3, 10e-3), precision="fp32_mixed"):
     """The C2 bunch inside a beam pipe along its whole path (64 x 256 quads): the loss test fused into
     the step loop (pp_push_lossy) with compaction of the live slots every 500 steps (partition + cell
-    sort), and without compaction; against the same steps with no mesh, through the generic kernel
-    with an alive mask and through the hot kernel.  A 3 mm pipe loses about 28 % of the bunch (sigma
+    sort), and without compaction; against the same steps with no mesh, with an alive mask (the hot
+    kernel's loss build, no test) and without.  A 3 mm pipe loses about 28 % of the bunch (sigma
     2 mm); a 10 mm pipe holds the bunch (the common case: the test runs, nothing is near a wall)."""
     from pypatools_amd.device import DeviceArray, synchronize
     from pypatools_amd.electrodes import TriangleElectrodes
@@ -335,7 +335,7 @@ def lossy_tracking(w, fp, qm, nsteps=2000, radii=(3e-3, 10e-3), precision="fp32_
 
     def measure():
         res = {"particles": M, "steps": nsteps, "precision": precision, "pipe_length_m": length,
-               "no_mesh_generic_kernel": run(None, 500), "no_mesh_hot_kernel": run(None, 500, alive=False),
+               "no_mesh_alive_mask": run(None, 500), "no_mesh_no_mask": run(None, 500, alive=False),
                "pipes": []}
         for radius, pipe in zip(radii, pipes):
             e = {"pipe_radius_m": radius, "triangles": pipe.n_triangles, "compacted": run(pipe, 500)}
@@ -851,7 +851,7 @@ def main():
     traffic, issue = None, {"status": cstatus}
     if rec is not None:
         if rec["steps_per_launch"] == steps_per_launch and rec.get("particles") == M:
-            traffic = rec["hbm_bytes_per_launch"]
+            traffic = rec.get("hbm_bytes_per_launch")       # recorded for FP32-mixed only
         wave_steps = -(-M // 64) * steps_per_launch
         slots = VALU_ISSUE_PER_S * avg_s                     # wave64 VALU issue slots in one launch
         issue.update({"valu_insts_per_wavefront_step": rec["valu_insts_per_wavefront_step"],
@@ -872,7 +872,7 @@ def main():
                    "precision": args.precision, "particles": M, "time_steps_per_bench_step": N},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / VALU_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true, false> (boris, "
+                     "kernel": f"pp_fast::k_hot<1, {str(args.precision == 'fp32_mixed').lower()}, true, false, false> (boris, "
                                f"periodic z; {steps_per_launch} steps/launch, avg {avg_ms:.3f} ms by HIP events)",
                      "flops_per_particle_step": FLOPS_PER_STEP, "particle_steps_per_launch": units,
                      "peak_note": "vector peak without packed FP32 (not used on this path) = FP64 vector peak",

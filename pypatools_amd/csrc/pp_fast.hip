@@ -837,10 +837,20 @@ constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || AL
 template <int ALG>
 constexpr int kHotWaves = (ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL) ? 4 : 6;
 
-template <int ALG, bool MIXED, bool PERIODIC, bool RF>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotWaves<ALG>))) k_hot(const HotParams p) {
+// LOSS: the state carries an alive mask (p.alive) and, with p.mesh, every step's segment is tested against
+// the electrode mesh inside the step loop (pp_push_lossy).  A particle dead at the start of the launch, or
+// lost in it, keeps its wavefront's lane: it repeats the step from its frozen state so the wavefront stays
+// full for the staged gathers, but it writes nothing; a particle lost in a step stores that step's state and
+// its record, as the generic kernels do (the reference marks it inactive after the step, pusher.py:982-990).
+// Every launch with an alive mask takes this build, with or without a mesh, so per-step launches tested on
+// the host and fused launches run the same instructions.
+template <int ALG, bool MIXED, bool PERIODIC, bool RF, bool LOSS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOSS ? 4 : kHotWaves<ALG>)))
+k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
+    bool live = true;
+    if constexpr (LOSS) live = p.alive[i] != 0;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p.m.coef), (short)0, (int)p.m.bytes, kBufferDword3);
     const AlgConst ac = hot_const(p);
@@ -849,6 +859,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotWa
     const double dt = p.dt;
     if (p.kick_only) {
         if constexpr (ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || ALG == PP_ALG_VAY) {
+            if (!live) return;
             if constexpr (MIXED)
                 v = kick_mixed<ALG>(i, r, v, dt, p.step0, HotFE32<PERIODIC, RF>{p.m, rs, hot_scale(p.m, 1.0), p.rf},
                                     ac);
@@ -869,20 +880,42 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotWa
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
     for (int64_t s = 0; s < p.nsteps; ++s) {
+        const d3 r0 = r, v0 = v;
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, fe64, ac);
+        if constexpr (LOSS) {
+            if (!live) {
+                r = r0;
+                v = v0;
+            }
+        }
         if (rec_on && --rec_left == 0) {
             rec_left = p.rec_every;
-            write_record_to(p.r_hist, p.v_hist, p.n, p.n_records, p.ids, rec_idx, i, r, vrec);
+            if (live) write_record_to(p.r_hist, p.v_hist, p.hist_rows, p.n_records, p.ids, rec_idx, i, r, vrec);
             ++rec_idx;
         }
+        if constexpr (LOSS) {
+            if (live && p.mesh &&
+                pp_bvh::mesh_hits(static_cast<const pp_bvh::MeshDev*>(p.mesh), r0.x, r0.y, r0.z, r.x, r.y, r.z)) {
+                live = false;
+                p.alive[i] = 0;
+                p.x[i] = r.x;
+                p.y[i] = r.y;
+                p.z[i] = r.z;
+                p.vx[i] = v.x;
+                p.vy[i] = v.y;
+                p.vz[i] = v.z;
+            }
+        }
     }
-    p.x[i] = r.x;
-    p.y[i] = r.y;
-    p.z[i] = r.z;
-    p.vx[i] = v.x;
-    p.vy[i] = v.y;
-    p.vz[i] = v.z;
+    if (live) {
+        p.x[i] = r.x;
+        p.y[i] = r.y;
+        p.z[i] = r.z;
+        p.vx[i] = v.x;
+        p.vy[i] = v.y;
+        p.vz[i] = v.z;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,21 +1163,24 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
     return hipGetLastError();
 }
 
-#define PP_HOT_LAUNCH(A, RFV)                                                                           \
+#define PP_HOT_LAUNCH(A, RFV, LV)                                                                       \
     if (mixed) {                                                                                        \
-        if (periodic) hipLaunchKernelGGL((k_hot<A, true, true, RFV>), g, b, 0, s, p);                  \
-        else hipLaunchKernelGGL((k_hot<A, true, false, RFV>), g, b, 0, s, p);                          \
+        if (periodic) hipLaunchKernelGGL((k_hot<A, true, true, RFV, LV>), g, b, 0, s, p);              \
+        else hipLaunchKernelGGL((k_hot<A, true, false, RFV, LV>), g, b, 0, s, p);                      \
     } else {                                                                                            \
-        if (periodic) hipLaunchKernelGGL((k_hot<A, false, true, RFV>), g, b, 0, s, p);                 \
-        else hipLaunchKernelGGL((k_hot<A, false, false, RFV>), g, b, 0, s, p);                         \
+        if (periodic) hipLaunchKernelGGL((k_hot<A, false, true, RFV, LV>), g, b, 0, s, p);             \
+        else hipLaunchKernelGGL((k_hot<A, false, false, RFV, LV>), g, b, 0, s, p);                     \
     }
 
+// launches with an alive mask (hot_eligible takes them without RF factors only) run the LOSS build
 #define PP_HOT_CASE(A)                                                                                  \
     case A:                                                                                             \
-        if (rf) {                                                                                       \
-            PP_HOT_LAUNCH(A, true)                                                                      \
+        if (loss) {                                                                                     \
+            PP_HOT_LAUNCH(A, false, true)                                                               \
+        } else if (rf) {                                                                                \
+            PP_HOT_LAUNCH(A, true, false)                                                               \
         } else {                                                                                        \
-            PP_HOT_LAUNCH(A, false)                                                                     \
+            PP_HOT_LAUNCH(A, false, false)                                                              \
         }                                                                                               \
         break;
 
@@ -1153,6 +1189,8 @@ hipError_t launch_hot(int alg, int prec, bool periodic, const HotParams& p, int 
     const dim3 g(grid_for(p.n, block)), b(block);
     const bool mixed = prec == PP_PREC_FP32_MIXED;
     const bool rf = p.rf.table != nullptr;
+    const bool loss = p.alive != nullptr;
+    if (loss && rf) return hipErrorInvalidValue;
     switch (alg) {
         PP_HOT_CASE(PP_ALG_LEAPFROG)
         PP_HOT_CASE(PP_ALG_BORIS)

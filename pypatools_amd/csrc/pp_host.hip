@@ -472,13 +472,13 @@ static void fill_hot_map(const MapDesc& m, const void* coef, uint64_t bytes, Hot
 // offset, every particle active, one dt, a single-gather algorithm or RK4, fast
 // or mixed precision.
 static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& hp) {
-    if (a->precision == PP_PREC_FP64 || !p.fused || a->alive || a->dt_pp) return false;
-    if (p.hist_rows != p.n) return false;   // the hot kernels write records of n rows
+    if (a->precision == PP_PREC_FP64 || !p.fused || a->dt_pp) return false;
     if (a->algorithm == PP_ALG_YOSHIDA || a->algorithm == PP_ALG_YOSHIDA_REL) return false;
     const FieldDev& E = p.E;
     const FieldDev& B = p.B;
     const MapDesc& m = E.map;
     if ((E.rf || B.rf) && !p.rf_table) return false;
+    if ((E.rf || B.rf) && a->alive) return false;   // the hot loss builds carry no RF factors
     if (E.has_offset || B.has_offset || E.comp0 != 0 || B.comp0 != 3 || m.ncomp != 6) return false;
     if (!m.uniform || m.n[0] < 2 || m.n[1] < 2 || m.n[2] < 2) return false;
     if (m.fill != 0.0) return false;        // the hot gathers read 0 outside the map
@@ -503,6 +503,9 @@ static bool hot_eligible(const pp_push_args* a, const PushParams& p, HotParams& 
     hp.v_hist = p.v_hist;
     hp.rec_every = p.rec_every;
     hp.n_records = p.n_records;
+    hp.hist_rows = p.hist_rows;
+    hp.alive = p.alive;
+    hp.mesh = p.mesh;
     hp.kick_only = p.kick_only;
     fill_hot_map(m, coef, bytes, hp.m);
     if (E.rf || B.rf) {

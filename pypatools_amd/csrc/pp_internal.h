@@ -99,6 +99,9 @@ struct HotParams {
     double dt, qm, c2, clight, vclamp;
     double* r_hist; double* v_hist;
     int64_t rec_every, n_records;
+    int64_t hist_rows;       // rows of a history record (>= n: a compacted state covers a live prefix)
+    uint8_t* alive;          // alive mask, or null (every particle live)
+    const void* mesh;        // device pp_bvh::MeshDev of the loss test, or null (with alive: no test)
     int32_t kick_only;
     int32_t _pad;
     HotMap m;

@@ -153,12 +153,16 @@ def test_track_with_device_losses_equals_host_callback_and_oracle(alg):
     assert 0.05 < 1 - act.mean() < 0.99                  # some, not all, particles were lost
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32_mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "fp64_fast", "fp32_mixed"])
 @pytest.mark.parametrize("alg", ["boris", "rk4_rel", "vay"])
 def test_lossy_track_through_a_map_with_compaction(alg, prec):
     """A beam pipe inside a 3-D map: the fused loss test with compaction every 7 steps (30 compactions,
     each a partition plus cell sort) equals the host-callback route step by step, histories and active
-    masks bit for bit; in strict FP64 it also equals the oracle pusher with the oracle mesh."""
+    masks bit for bit (in FP32-mixed and FP64-fast both routes run the hot kernel's loss build); in
+    strict FP64 it also equals the oracle pusher with the oracle mesh, and in the fast precisions it
+    follows the oracle to their tolerance: the same particles are lost (a particle whose segment grazes
+    the wall may go either way: at most 0.5 % differ) and the survivors' final positions agree to 1e-6
+    (FP32-mixed) or 1e-11 (FP64-fast) of the path."""
     rng = np.random.default_rng(52)
     n = 5000
     x = (np.arange(16) - 7.5) * 1e-3
@@ -181,13 +185,22 @@ def test_lossy_track_through_a_map_with_compaction(alg, prec):
     np.testing.assert_array_equal(rh, rh2)
     np.testing.assert_array_equal(vh, vh2)
     assert 0.1 < 1 - act.mean() < 0.95
+    orc = OraclePusher(ion.q_over_m, alg)
+    # particles are independent: the oracle runs on the first k of them (all for the bit-for-bit FP64 check)
+    k = n if prec == "fp64" else 800
+    orh, ovh, oact = orc.track_batch(r0[:k], v0[:k], OField.from_arrays(grid, E), OField.from_arrays(grid, B), 210,
+                                     5e-11, 30, electrode_assembly=orc_e)
     if prec == "fp64":
-        orc = OraclePusher(ion.q_over_m, alg)
-        orh, ovh, oact = orc.track_batch(r0, v0, OField.from_arrays(grid, E), OField.from_arrays(grid, B), 210,
-                                         5e-11, 30, electrode_assembly=orc_e)
         np.testing.assert_array_equal(act, oact)
         np.testing.assert_array_equal(rh, orh)
         np.testing.assert_array_equal(vh, ovh)
+    else:
+        assert np.mean(act[:k] != oact) <= 5e-3, np.mean(act[:k] != oact)
+        both = act[:k] & oact
+        assert 0.1 < 1 - oact.mean() < 0.95
+        path = np.linalg.norm(v0[:k][both], axis=1) * 210 * 5e-11
+        err = np.max(np.linalg.norm(rh[-1][:k][both] - orh[-1][both], axis=1) / path)
+        assert err < (1e-6 if prec == "fp32_mixed" else 1e-11), err
 
 
 def test_every_particle_lost_ends_the_run_early():
