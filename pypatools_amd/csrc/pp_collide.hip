@@ -324,10 +324,25 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
             d.clear = m->clear;
             d.nnodes = m->nnodes;
             d.nlevels = cand.nlevels;
+            // FP32 cell estimate of the pre-test (clear_ahead): its error, in cells, is at most
+            // 2^-24 (2 X inv + |lo| inv + n0) with X the largest |coordinate| of the grid box
+            // (rounding of x, of inv, of -lo inv, of the fma); the device takes twice that as
+            // ferr.  Where it is not below a quarter cell the offset is set so that no point
+            // passes the pre-test's range check.
+            double ferr = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double X = std::max(std::fabs(glo[a]), std::fabs(ghi[a]));
+                ferr = std::max(ferr, std::ldexp(2.0 * X * cand.inv[a] + std::fabs(cand.lo[a]) * cand.inv[a] +
+                                                 (double)cand.n0[a], -23));
+            }
+            const bool fp32_ok = ferr < 0.25;
+            d.ferr = fp32_ok ? (float)ferr : 0.25f;
             for (int a = 0; a < 3; ++a) {
                 d.n0[a] = cand.n0[a];
                 d.lo[a] = cand.lo[a];
                 d.inv[a] = cand.inv[a];
+                d.sf[a] = (float)cand.inv[a];
+                d.of[a] = fp32_ok ? (float)(-cand.lo[a] * cand.inv[a]) : -1e30f;
             }
             for (int l = 0; l < pp_bvh::kMaxLevels; ++l) d.off[l] = cand.off[l];
             d.margin = bld.margin;

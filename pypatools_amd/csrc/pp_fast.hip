@@ -879,8 +879,13 @@ k_hot(const HotParams p) {
     int64_t rec_left = p.rec_every - (p.step0 % p.rec_every);
     int64_t rec_idx = p.step0 / p.rec_every + 1;
     d3 vrec = v;
+    const pp_bvh::MeshDev* mesh = static_cast<const pp_bvh::MeshDev*>(p.mesh);
     for (int64_t s = 0; s < p.nsteps; ++s) {
         const d3 r0 = r, v0 = v;
+        pp_bvh::ClearProbe probe{0, false, false};   // read before the step so its latency is hidden
+        if constexpr (LOSS) {
+            if (mesh) probe = pp_bvh::mesh_clearance(mesh, r.x, r.y, r.z);
+        }
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, fe64, ac);
         if constexpr (LOSS) {
@@ -895,8 +900,7 @@ k_hot(const HotParams p) {
             ++rec_idx;
         }
         if constexpr (LOSS) {
-            if (live && p.mesh &&
-                pp_bvh::mesh_hits(static_cast<const pp_bvh::MeshDev*>(p.mesh), r0.x, r0.y, r0.z, r.x, r.y, r.z)) {
+            if (live && mesh && pp_bvh::mesh_hits_fetched(mesh, probe, r0.x, r0.y, r0.z, r.x, r.y, r.z)) {
                 live = false;
                 p.alive[i] = 0;
                 p.x[i] = r.x;

@@ -164,7 +164,7 @@ def test_lossy_track_through_a_map_with_compaction(alg, prec):
     the wall may go either way: at most 0.5 % differ) and the survivors' final positions agree to 1e-6
     (FP32-mixed) or 1e-11 (FP64-fast) of the path."""
     rng = np.random.default_rng(52)
-    n = 5000
+    n = 2500
     x = (np.arange(16) - 7.5) * 1e-3
     z = np.arange(16) * 1e-3 - 2e-3
     grid = {"x": x, "y": x, "z": z}
