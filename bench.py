@@ -421,7 +421,26 @@ def cpu_baseline(w, seconds=15.0):
             "as_shipped": {"value": Ms * steps2 / el2, "unit": "particle-steps/s", "cores": 1,
                            "sample": f"{Ms} protons x {steps2} steps of the reference's no-numba structure "
                                      f"(vectorised gather, per-particle Boris kernel, libm pow squares), "
-                                     f"{el2:.1f} s"}}
+                                     f"{el2:.1f} s"},
+            "host": host_info()}
+
+
+def host_info():
+    """The host the CPU baseline ran on: CPU model, os.cpu_count() and the process's affinity size."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu_model": model, "cpu_count": os.cpu_count(), "affinity": affinity}
 
 
 def run_c4(args, reps=None, warmup=None):
@@ -557,6 +576,35 @@ def bench_c3(args):
     print(json.dumps(out), flush=True)
 
 
+def c2_algorithms(w, fp, qm, args):
+    """The C2 track with the other pushers of the C2 config (textbook vay, the reference's vay_rel): the rate in
+    FP32-mixed and strict FP64, and the FP32-mixed position error against strict FP64 (= the oracle)."""
+    from pypatools_amd.device import synchronize
+    from pypatools_amd.engine import U_STATE, ParticleState, Tracker
+    M, N, dt = len(w["r0"]), args.nsteps, w["dt"]
+    path = np.linalg.norm(w["v0"], axis=1) * N * dt
+    out = {}
+    for alg in ("vay", "vay_rel"):
+        res, pos = {}, {}
+        for prec in ("fp32_mixed", "fp64"):
+            st = ParticleState.from_host(w["r0"], w["v0"])
+            if alg in U_STATE:
+                st.v_to_u()
+            tr = Tracker(alg, prec, qm, block_size=args.block, sort_every=args.sort_every,
+                         refine_bits=args.refine_bits, refine_axis=2)
+            synchronize()
+            t = time.perf_counter()
+            tr.run_resident(st, fp, N, dt)
+            synchronize()
+            res[f"{prec}_particle_steps_per_s"] = M * N / (time.perf_counter() - t)
+            st.unsort()
+            pos[prec] = st.to_host()[0]
+            del st
+        res["pos_err_rel_path_max"] = float(np.max(np.linalg.norm(pos["fp32_mixed"] - pos["fp64"], axis=1) / path))
+        out[alg] = res
+    return out
+
+
 def bench_c1(args):
     """C1: FP64 Boris in a uniform 1 T field, 1e4 protons x 1000 steps; error vs the closed-form gyro-orbit."""
     import pypatools_amd as pa
@@ -579,13 +627,30 @@ def bench_c1(args):
     exact = np.column_stack([rx, ry, rz])
     path = np.linalg.norm(v0, axis=1) * N * dt
     err = float(np.max(np.linalg.norm(r - exact, axis=1) / path))
+    # boris_rel: the state holds u = gamma v; in pure B gamma is constant and u rotates by
+    # theta_rel = 2 atan(qm B dt / (2 gamma)) per step, the position advancing by dt u / gamma
+    from pypatools_amd.constants import CLIGHT
+    st_rel = ParticleState.from_host(r0, v0)
+    st_rel.v_to_u()
+    rate_rel, (r_rel, _) = _track_rate(Tracker("boris_rel", "fp64", qm), st_rel, fp, N, dt, reps=args.steps)
+    gam = 1.0 / np.sqrt(1.0 - np.sum(v0 * v0, axis=1) / CLIGHT ** 2)
+    th_r = 2.0 * np.arctan(qm * 1.0 * dt / (2.0 * gam))
+    nn = np.arange(1, N + 1)[None, :]
+    c_r, s_r = np.cos(nn * th_r[:, None]).sum(axis=1), np.sin(nn * th_r[:, None]).sum(axis=1)
+    exact_rel = np.column_stack([r0[:, 0] + dt * (c_r * v0[:, 0] + s_r * v0[:, 1]),
+                                 r0[:, 1] + dt * (-s_r * v0[:, 0] + c_r * v0[:, 1]), r0[:, 2] + dt * N * v0[:, 2]])
+    err_rel = float(np.max(np.linalg.norm(r_rel - exact_rel, axis=1) / path))
+    larmor = float(np.mean(np.linalg.norm(v0[:, :2], axis=1) / (qm * 1.0)))
     out = {"metric": "particle-steps/sec on 1 MI355X (C1 plain FP64 Boris, uniform B)", "value": value,
            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
            "ms_per_step": 1e3 * len(r0) * N / value, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "C1: 1e4 protons at 1 MeV, isotropic, uniform B = 1 T along z, E = 0, "
                                   "dt 1e-11 s, 1000 steps, boris (FP64)", "precision": "fp64"},
-           "extra": {"pos_err_rel_path_max_vs_closed_form": err, "theta_per_step": th}}
+           "extra": {"pos_err_rel_path_max_vs_closed_form": err, "theta_per_step": th,
+                     "boris_rel": {"particle_steps_per_s": rate_rel, "pos_err_rel_path_max_vs_closed_form": err_rel,
+                                   "theta_per_step_mean": float(np.mean(th_r))},
+                     "mean_transverse_larmor_radius_m": larmor}}
     print(json.dumps(out), flush=True)
 
 
@@ -806,6 +871,7 @@ def main():
                 extra["pos_err_max_m"] = float(np.max(dr))
                 extra["pos_err_denominator"] = "|v0| N dt (path length); FP64 strict result = oracle bit for bit"
             del s64
+        extra["c2_algorithms"] = c2_algorithms(w, fp, qm, args)
 
     # ---- warmup + timed bench steps (each restarts the track; inputs resident in HBM)
     for _ in range(args.warmup):
