@@ -836,6 +836,11 @@ constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || AL
 // spills grow with the occupancy).
 template <int ALG>
 constexpr int kHotWaves = (ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL) ? 4 : 6;
+// The loss builds hold the mesh test's state as well: 5 wavefronts per SIMD (96 VGPRs) for the FP32-mixed
+// single-gather steps (no spills in the step loop; the 10 mm pipe ran 7.3e10 against 7.0e10 at 4), 4 for
+// FP64-fast and the RK4 steps.
+template <int ALG, bool MIXED, bool LOSS>
+constexpr int kHotLossWaves = !LOSS ? kHotWaves<ALG> : (MIXED && kHotWaves<ALG> > 4 ? 5 : 4);
 
 // LOSS: the state carries an alive mask (p.alive) and, with p.mesh, every step's segment is tested against
 // the electrode mesh inside the step loop (pp_push_lossy).  A particle dead at the start of the launch, or
@@ -845,7 +850,7 @@ constexpr int kHotWaves = (ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL) ? 4 : 6;
 // Every launch with an alive mask takes this build, with or without a mesh, so per-step launches tested on
 // the host and fused launches run the same instructions.
 template <int ALG, bool MIXED, bool PERIODIC, bool RF, bool LOSS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOSS ? 4 : kHotWaves<ALG>)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotLossWaves<ALG, MIXED, LOSS>)))
 k_hot(const HotParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
