@@ -460,7 +460,7 @@ def run_c4(args, reps=None, warmup=None):
     if args.autotune:
         sub = bunches[::8]
         t = time.perf_counter()
-        cfg = autotune_batch(fields, sub, args.precision, sample_steps=600, use_cache=False)
+        cfg = autotune_batch(fields, sub, args.precision, sample_steps=1000, use_cache=False)
         tune = {"autotune_s": time.perf_counter() - t, "config": cfg}
     else:
         cfg = {"block_size": 256, "steps_per_launch": 500, "sort_every": 500}

@@ -87,7 +87,13 @@ def autotune_batch(maps, bunches, precision="fp32_mixed", sample_steps=1000, use
         return t
 
     best_t = trial(best)
-    for key, values in (("sort_every", SORT_EVERY), ("block_size", BLOCKS), ("steps_per_launch", STEPS_PER_LAUNCH)):
+    # a sample must show what a candidate costs over a long run: a sort cadence is tried only where the
+    # sample holds at least two of its sorts (never sorting only where it spans 4000 steps), a launch
+    # length only where the sample holds one launch of it; otherwise a cadence that skips its sorts in a
+    # short sample looks cheapest and then slows the long run
+    sorts = tuple(v for v in SORT_EVERY if (v == 0 and sample_steps >= 4000) or (v > 0 and 2 * v <= sample_steps))
+    lengths = tuple(v for v in STEPS_PER_LAUNCH if v <= sample_steps)
+    for key, values in (("sort_every", sorts), ("block_size", BLOCKS), ("steps_per_launch", lengths)):
         for val in values:
             if val == best[key]:
                 continue
