@@ -401,33 +401,16 @@ DEV double2 ld2d(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return d;
 }
 
-// Trilinear monomials of the cell-local coordinates, shared by the 6 components.
-struct Mono {
-    float x, y, z, xy, xz, yz, xyz;
-};
-
-DEV Mono monomials(float tx, float ty, float tz) {
-    Mono w;
-    w.x = tx;
-    w.y = ty;
-    w.z = tz;
-    w.xy = tx * ty;
-    w.xz = tx * tz;
-    w.yz = ty * tz;
-    w.xyz = w.xy * tz;
-    return w;
-}
-
-// One component from its 8 coefficients (of 1, x, y, z, xy, xz, yz, xyz) as a
-// chain of 7 FMAs; the scalar-cache path below evaluates the same chain.
-DEV float ceval(float4 lo, float4 hi, const Mono& w) {
-    float r = fmaf(lo.y, w.x, lo.x);
-    r = fmaf(lo.z, w.y, r);
-    r = fmaf(lo.w, w.z, r);
-    r = fmaf(hi.x, w.xy, r);
-    r = fmaf(hi.y, w.xz, r);
-    r = fmaf(hi.z, w.yz, r);
-    return fmaf(hi.w, w.xyz, r);
+// One component from its 8 coefficients (of 1, x, y, z, xy, xz, yz, xyz) in nested form,
+// f = (c0 + ty (cy + tz cyz) + tz cz) + tx (cx + ty (cxy + tz cxyz) + tz cxz): 7 FMAs with no
+// products of the weights, and only two of them take two coefficients (an FMA reads one scalar
+// register, so the scalar-cache path below moves one coefficient of each of those two into a
+// vector register).  Every route to the coefficients evaluates this same chain.
+DEV float ceval(float4 lo, float4 hi, float tx, float ty, float tz) {
+    const float a = fmaf(ty, fmaf(tz, hi.w, hi.x), lo.y);
+    const float b = fmaf(tz, hi.y, a);
+    const float d = fmaf(tz, lo.w, fmaf(ty, fmaf(tz, hi.z, lo.z), lo.x));
+    return fmaf(tx, b, d);
 }
 
 DEV double cevald(double2 ab, double2 cd, double2 ef, double2 gh, double tx, double ty, double tz) {
@@ -441,7 +424,9 @@ DEV double cevald(double2 ab, double2 cd, double2 ef, double2 gh, double tx, dou
 
 // cell index (an integer-valued double) and weight on one non-periodic axis, u-space bounds test.
 // Only points inside (ok) use the index, and there floor(u) >= 0: it is clamped at the top alone
-// (u = nc lies in the last cell).
+// (u = nc lies in the last cell).  Inside, u - fu is exact and lies in [0, 1]; the clamp (the
+// subtraction's output modifier, no instruction of its own) keeps the weights of points outside
+// finite, where the coefficients read as zero.
 DEV double hot_axis(double x, double id, double c, int nc, double& t, bool& ok) {
     const double u = fma(x, id, c);
     ok = ok && (u >= 0.0) && (u <= (double)nc);
@@ -594,14 +579,11 @@ DEV uint32_t* stage_list() {
 // address become scalar loads (s_load_dwordx16) into SGPRs.
 typedef const __attribute__((address_space(4))) float* cfloat_p;
 
-DEV float ceval_s(cfloat_p q, const Mono& w) {
-    float r = fmaf(q[1], w.x, q[0]);
-    r = fmaf(q[2], w.y, r);
-    r = fmaf(q[3], w.z, r);
-    r = fmaf(q[4], w.xy, r);
-    r = fmaf(q[5], w.xz, r);
-    r = fmaf(q[6], w.yz, r);
-    return fmaf(q[7], w.xyz, r);
+DEV float ceval_s(cfloat_p q, float tx, float ty, float tz) {
+    const float a = fmaf(ty, fmaf(tz, q[7], q[4]), q[1]);
+    const float b = fmaf(tz, q[5], a);
+    const float d = fmaf(tz, q[3], fmaf(ty, fmaf(tz, q[6], q[2]), q[0]));
+    return fmaf(tx, b, d);
 }
 #endif
 
@@ -686,7 +668,7 @@ struct HotFE32 {
         bool ok;
         uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
         cell = ok ? cell : kOutside32;
-        const Mono w = monomials((float)tx, (float)ty, (float)tz);
+        const float wx = (float)tx, wy = (float)ty, wz = (float)tz;
         bool done = false;
 #if defined(__HIP_DEVICE_COMPILE__)
         constexpr int kPitch = StageLanes<12>::kPitch;
@@ -695,7 +677,7 @@ struct HotFE32 {
             if (staged && __builtin_amdgcn_ballot_w64(cell != last_cell) == 0) {
                 const float4* q = wv + __umul24(last_slot, (uint32_t)kPitch);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
+                for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], wx, wy, wz);
                 done = true;
             }
         }
@@ -712,7 +694,7 @@ struct HotFE32 {
         } else if (__builtin_amdgcn_ballot_w64(cell != c0) == 0 && c0 < m.ncells) {
             const cfloat_p q = (cfloat_p)m.coef + (size_t)c0 * 48u;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, w);
+            for (int c = 0; c < 6; ++c) v[c] = ceval_s(q + c * 8, wx, wy, wz);
             done = true;
         } else if (__builtin_amdgcn_ballot_w64(true) == ~0ull) {
             uint32_t myslot = 0;
@@ -721,7 +703,7 @@ struct HotFE32 {
                 stage_cells<12>(wv, stage_list(), rs, 192u, ns, cell, myslot, lanes);
                 const float4* q = wv + __umul24(myslot, (uint32_t)kPitch);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], w);
+                for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], wx, wy, wz);
                 __builtin_amdgcn_wave_barrier();
                 done = true;
                 if constexpr (REUSE) {
@@ -736,7 +718,7 @@ struct HotFE32 {
             const uint32_t base = __umul24(cell, 48u * 4u);
 #pragma unroll
             for (int c = 0; c < 6; ++c)
-                v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), w);
+                v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), wx, wy, wz);
         }
     }
 };
@@ -965,11 +947,13 @@ DEV void batch_steps(int64_t i, d3& r, d3& v, int64_t step0, int64_t ns, double 
     }
 }
 
-// Kernel per tile class (pp_tile_class): the single-gather algorithms alone fit 6 wavefronts per SIMD,
-// like k_hot; a kernel holding the RK4 steps keeps 4 (C4, Boris and Vay bunches only: 1.62e11 particle-steps/s
-// with every algorithm in one kernel at 4, 1.90e11 at 6; the C5 mix with its RK4 jobs ran worse at 6).
-template <int CLASS>
-constexpr int kBatchWaves = CLASS == PP_TILES_SINGLE_GATHER ? 6 : 4;
+// Kernel per tile class (pp_tile_class): the single-gather algorithms alone fit 7 wavefronts per SIMD in
+// FP32-mixed (72 VGPRs with no scratch access in the step loops; 6 in FP64-fast); a kernel holding the RK4
+// steps keeps 4.  Measured on C4 (Boris and Vay bunches only): 1.62e11 particle-steps/s with every algorithm
+// in one kernel at 4, 1.90e11 at 6 (whose allocation spilled inside the step loops), 2.03e11 at 7, 1.98e11
+// at 8; the C5 mix with its RK4 jobs ran worse with the RK4 steps at 6.
+template <bool MIXED, int CLASS>
+constexpr int kBatchWaves = CLASS != PP_TILES_SINGLE_GATHER ? 4 : (MIXED ? 7 : 6);
 
 template <int CLASS>
 constexpr bool batch_class_has(int alg) {
@@ -978,7 +962,7 @@ constexpr bool batch_class_has(int alg) {
 }
 
 template <bool MIXED, bool PERIODIC, int CLASS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBatchWaves<CLASS>)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBatchWaves<MIXED, CLASS>)))
 k_batched(const BatchParams p) {
     const int64_t t = blockIdx.x;
     if (t >= p.ntiles) return;
