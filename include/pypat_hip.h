@@ -307,7 +307,8 @@ typedef struct pp_batch_args {
     int32_t _pad;
 } pp_batch_args;
 /* PP_TILES_ALL: one kernel for every algorithm (allocated for 4 wavefronts per SIMD);
-   PP_TILES_SINGLE_GATHER: boris, vay, vay_rel, boris_rel, leapfrog (6 wavefronts per SIMD);
+   PP_TILES_SINGLE_GATHER: boris, vay, vay_rel, boris_rel, leapfrog (7 wavefronts per SIMD in
+   FP32-mixed, 6 in FP64-fast);
    PP_TILES_RK4: rk4, rk4_rel (4 wavefronts per SIMD).  Callers with both kinds of bunches pass
    each launch the tiles of one class. */
 enum pp_tile_class { PP_TILES_ALL = 0, PP_TILES_SINGLE_GATHER = 1, PP_TILES_RK4 = 2 };
