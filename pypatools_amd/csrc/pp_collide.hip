@@ -43,6 +43,8 @@ struct pp_mesh {
     pp_bvh::Tri* tris;
     pp_bvh::Range* ranges;  // candidate grids, all levels
     uint8_t* clear;         // level-0 clearance (MeshDev::clear)
+    int32_t* cells;         // level-0 triangle lists (MeshDev::cells, MeshDev::cell_tris), or null
+    int32_t* cell_tris;
     pp_bvh::MeshDev* dev;   // device copy of the descriptor
     int32_t nnodes;
     int64_t ntris;
@@ -127,6 +129,47 @@ struct Candidates {
     int64_t off[pp_bvh::kMaxLevels] = {};
     std::vector<pp_bvh::Range> ranges;
     std::vector<uint8_t> clear;   // level 0: Chebyshev distance to the nearest non-empty cell, capped at 255
+    // Level-0 triangle lists (MeshDev::cells, MeshDev::cell_tris): the triangles (positions in the
+    // sorted order) whose grown box overlaps each cell.  A cell with more than kCellTris of them has
+    // no list (start word with bit 31 set); all lists are dropped when they would hold more than
+    // kListRefs entries.
+    std::vector<int32_t> cells, cell_tris;
+    static constexpr int32_t kCellTris = 32;
+    static constexpr int64_t kListRefs = (int64_t)1 << 24;
+
+    // c0, c1: the level-0 cell box of each triangle (3 each, in sorted order)
+    void build_lists(const std::vector<int32_t>& c0, const std::vector<int32_t>& c1, int64_t nt) {
+        const int64_t total = (int64_t)n0[0] * n0[1] * n0[2];
+        std::vector<int32_t> cnt((size_t)total, 0);
+        for (int64_t k = 0; k < nt; ++k)
+            for (int32_t i = c0[3 * k]; i <= c1[3 * k]; ++i)
+                for (int32_t j = c0[3 * k + 1]; j <= c1[3 * k + 1]; ++j)
+                    for (int32_t q = c0[3 * k + 2]; q <= c1[3 * k + 2]; ++q) {
+                        int32_t& c = cnt[((size_t)i * n0[1] + j) * n0[2] + q];
+                        if (c <= kCellTris) ++c;
+                    }
+        cells.assign((size_t)total + 1, 0);
+        int64_t refs = 0;
+        for (int64_t c = 0; c < total; ++c) {
+            const bool listed = cnt[(size_t)c] <= kCellTris;
+            cells[(size_t)c] = (int32_t)refs | (listed ? 0 : (int32_t)0x80000000u);
+            if (listed) refs += cnt[(size_t)c];
+            if (refs > kListRefs) {
+                cells.clear();
+                return;
+            }
+        }
+        cells[(size_t)total] = (int32_t)refs;
+        cell_tris.assign((size_t)refs, 0);
+        std::vector<int32_t> fill((size_t)total, 0);
+        for (int64_t k = 0; k < nt; ++k)
+            for (int32_t i = c0[3 * k]; i <= c1[3 * k]; ++i)
+                for (int32_t j = c0[3 * k + 1]; j <= c1[3 * k + 1]; ++j)
+                    for (int32_t q = c0[3 * k + 2]; q <= c1[3 * k + 2]; ++q) {
+                        const size_t c = ((size_t)i * n0[1] + j) * n0[2] + q;
+                        if (cells[c] >= 0) cell_tris[(size_t)(cells[c] + fill[c]++)] = (int32_t)k;
+                    }
+    }
 
     // Multi-source breadth-first search over the 26 neighbours of each cell: a
     // search front that advances one step per ring reaches a cell at exactly its
@@ -211,9 +254,11 @@ struct Candidates {
                 for (int32_t k = 0; k < bld.nodes[v].count; ++k) leaf_of[bld.nodes[v].first + k] = v;
         // level 0: common ancestor of the triangles near each cell
         std::vector<int32_t> cur((size_t)n0[0] * n0[1] * n0[2], -1);
+        std::vector<int32_t> tc0(3 * (size_t)nt), tc1(3 * (size_t)nt);
         for (int64_t k = 0; k < nt; ++k) {
             const int32_t t = bld.order[k];
-            int32_t c0[3], c1[3];
+            int32_t* c0 = &tc0[3 * (size_t)k];
+            int32_t* c1 = &tc1[3 * (size_t)k];
             for (int a = 0; a < 3; ++a) {
                 c0[a] = cell(a, bld.lo[3 * t + a] - bld.margin);
                 c1[a] = cell(a, bld.hi[3 * t + a] + bld.margin);
@@ -226,6 +271,7 @@ struct Candidates {
                     }
         }
         build_clearance(cur);
+        build_lists(tc0, tc1, nt);
         int32_t n[3] = {n0[0], n0[1], n0[2]};
         ranges.clear();
         for (nlevels = 0; nlevels < pp_bvh::kMaxLevels; ++nlevels) {
@@ -306,6 +352,8 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
     m->tris = nullptr;
     m->ranges = nullptr;
     m->clear = nullptr;
+    m->cells = nullptr;
+    m->cell_tris = nullptr;
     m->dev = nullptr;
     m->range_cells = nt > 0 ? (int64_t)cand.ranges.size() : 0;
     hipGetDevice(&m->device);
@@ -317,11 +365,19 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
                   hipMalloc(&m->ranges, rbytes) == hipSuccess &&
                   hipMalloc(&m->clear, cand.clear.size()) == hipSuccess &&
                   hipMalloc(&m->dev, sizeof(pp_bvh::MeshDev)) == hipSuccess;
+        const bool lists = !cand.cells.empty();
+        if (ok && lists)
+            ok = hipMalloc(&m->cells, sizeof(int32_t) * cand.cells.size()) == hipSuccess &&
+                 (cand.cell_tris.empty() ||
+                  hipMalloc(&m->cell_tris, sizeof(int32_t) * cand.cell_tris.size()) == hipSuccess);
         if (ok) {
             d.nodes = m->nodes;
             d.tris = m->tris;
             d.ranges = m->ranges;
             d.clear = m->clear;
+            d.cells = m->cells;
+            d.cell_tris = m->cell_tris;
+            d.lists = lists ? 1 : 0;
             d.nnodes = m->nnodes;
             d.nlevels = cand.nlevels;
             // FP32 cell estimate of the pre-test (clear_ahead): its error, in cells, is at most
@@ -350,7 +406,12 @@ int create(const double* v, int64_t nv, const int32_t* idx, int64_t nt, pp_mesh*
                  hipMemcpy(m->tris, sorted.data(), sizeof(Tri) * (size_t)nt, hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemcpy(m->ranges, cand.ranges.data(), rbytes, hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemcpy(m->clear, cand.clear.data(), cand.clear.size(), hipMemcpyHostToDevice) == hipSuccess &&
-                 hipMemcpy(m->dev, &d, sizeof(d), hipMemcpyHostToDevice) == hipSuccess;
+                 hipMemcpy(m->dev, &d, sizeof(d), hipMemcpyHostToDevice) == hipSuccess &&
+                 (!lists || hipMemcpy(m->cells, cand.cells.data(), sizeof(int32_t) * cand.cells.size(),
+                                      hipMemcpyHostToDevice) == hipSuccess) &&
+                 (!m->cell_tris || hipMemcpy(m->cell_tris, cand.cell_tris.data(),
+                                             sizeof(int32_t) * cand.cell_tris.size(),
+                                             hipMemcpyHostToDevice) == hipSuccess);
         }
         if (!ok) {
             destroy(m);
@@ -368,6 +429,8 @@ void destroy(pp_mesh* m) {
     if (m->tris) (void)hipFree(m->tris);
     if (m->ranges) (void)hipFree(m->ranges);
     if (m->clear) (void)hipFree(m->clear);
+    if (m->cells) (void)hipFree(m->cells);
+    if (m->cell_tris) (void)hipFree(m->cell_tris);
     if (m->dev) (void)hipFree(m->dev);
     delete m;
 }

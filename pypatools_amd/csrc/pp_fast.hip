@@ -818,22 +818,24 @@ constexpr bool kPrescaled = ALG == PP_ALG_BORIS || ALG == PP_ALG_BORIS_REL || AL
 // spills grow with the occupancy).
 template <int ALG>
 constexpr int kHotWaves = (ALG == PP_ALG_RK4 || ALG == PP_ALG_RK4_REL) ? 4 : 6;
-// The loss builds hold the mesh test's state as well: 5 wavefronts per SIMD (96 VGPRs) for the FP32-mixed
-// single-gather steps (no spills in the step loop; the 10 mm pipe ran 7.3e10 against 7.0e10 at 4), 4 for
-// FP64-fast and the RK4 steps.
+// The loss builds without a mesh hold the frozen state as well: 5 wavefronts per SIMD (88 VGPRs, no
+// spills) for the FP32-mixed single-gather steps, 4 for FP64-fast and the RK4 steps.  The mesh builds
+// (k_hot_mesh) take 4 everywhere: at 5 the FP32-mixed one spilled 20 VGPRs (C2 bunch, 10 mm pipe 8.4e10
+// particle-steps/s against 8.7e10 at 4; 3 mm pipe 3.85e10 at both).
 template <int ALG, bool MIXED, bool LOSS>
 constexpr int kHotLossWaves = !LOSS ? kHotWaves<ALG> : (MIXED && kHotWaves<ALG> > 4 ? 5 : 4);
+constexpr int kHotMeshWaves = 4;
 
-// LOSS: the state carries an alive mask (p.alive) and, with p.mesh, every step's segment is tested against
-// the electrode mesh inside the step loop (pp_push_lossy).  A particle dead at the start of the launch, or
+// LOSS: the state carries an alive mask (p.alive); MESH (k_hot_mesh): every step's segment is also tested
+// against the electrode mesh p.mesh inside the step loop (pp_push_lossy).  A particle dead at the start of the launch, or
 // lost in it, keeps its wavefront's lane: it repeats the step from its frozen state so the wavefront stays
 // full for the staged gathers, but it writes nothing; a particle lost in a step stores that step's state and
 // its record, as the generic kernels do (the reference marks it inactive after the step, pusher.py:982-990).
-// Every launch with an alive mask takes this build, with or without a mesh, so per-step launches tested on
-// the host and fused launches run the same instructions.
-template <int ALG, bool MIXED, bool PERIODIC, bool RF, bool LOSS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotLossWaves<ALG, MIXED, LOSS>)))
-k_hot(const HotParams p) {
+// Every launch with an alive mask takes a loss build, k_hot_mesh with a mesh and k_hot<..., true> without:
+// both run the same step arithmetic, so per-step launches tested on the host and fused launches agree bit
+// for bit, and the mesh test's registers stay out of the build without it.
+template <int ALG, bool MIXED, bool PERIODIC, bool RF, bool LOSS, bool MESH>
+DEV void hot_body(const HotParams& p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     bool live = true;
@@ -870,9 +872,7 @@ k_hot(const HotParams p) {
     for (int64_t s = 0; s < p.nsteps; ++s) {
         const d3 r0 = r, v0 = v;
         pp_bvh::ClearProbe probe{0, false, false};   // read before the step so its latency is hidden
-        if constexpr (LOSS) {
-            if (mesh) probe = pp_bvh::mesh_clearance(mesh, r.x, r.y, r.z);
-        }
+        if constexpr (MESH) probe = pp_bvh::mesh_clearance(mesh, r.x, r.y, r.z);
         if constexpr (MIXED) step_mixed<ALG, PRE>(i, r, v, vrec, dt, h, p.step0 + s, fe32, ac);
         else step64<ALG>(i, r, v, vrec, dt, p.step0 + s, fe64, ac);
         if constexpr (LOSS) {
@@ -886,8 +886,8 @@ k_hot(const HotParams p) {
             if (live) write_record_to(p.r_hist, p.v_hist, p.hist_rows, p.n_records, p.ids, rec_idx, i, r, vrec);
             ++rec_idx;
         }
-        if constexpr (LOSS) {
-            if (live && mesh && pp_bvh::mesh_hits_fetched(mesh, probe, r0.x, r0.y, r0.z, r.x, r.y, r.z)) {
+        if constexpr (MESH) {
+            if (live && pp_bvh::mesh_hits_fetched(mesh, probe, r0.x, r0.y, r0.z, r.x, r.y, r.z)) {
                 live = false;
                 p.alive[i] = 0;
                 p.x[i] = r.x;
@@ -907,6 +907,18 @@ k_hot(const HotParams p) {
         p.vy[i] = v.y;
         p.vz[i] = v.z;
     }
+}
+
+template <int ALG, bool MIXED, bool PERIODIC, bool RF, bool LOSS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotLossWaves<ALG, MIXED, LOSS>)))
+k_hot(const HotParams p) {
+    hot_body<ALG, MIXED, PERIODIC, RF, LOSS, false>(p);
+}
+
+template <int ALG, bool MIXED, bool PERIODIC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHotMeshWaves)))
+k_hot_mesh(const HotParams p) {
+    hot_body<ALG, MIXED, PERIODIC, false, true, true>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,10 +1177,21 @@ hipError_t launch_push(int alg, int prec, const PushParams& p, int block, hipStr
         else hipLaunchKernelGGL((k_hot<A, false, false, RFV, LV>), g, b, 0, s, p);                     \
     }
 
-// launches with an alive mask (hot_eligible takes them without RF factors only) run the LOSS build
+#define PP_HOT_MESH_LAUNCH(A)                                                                           \
+    if (mixed) {                                                                                        \
+        if (periodic) hipLaunchKernelGGL((k_hot_mesh<A, true, true>), g, b, 0, s, p);                  \
+        else hipLaunchKernelGGL((k_hot_mesh<A, true, false>), g, b, 0, s, p);                          \
+    } else {                                                                                            \
+        if (periodic) hipLaunchKernelGGL((k_hot_mesh<A, false, true>), g, b, 0, s, p);                 \
+        else hipLaunchKernelGGL((k_hot_mesh<A, false, false>), g, b, 0, s, p);                         \
+    }
+
+// launches with an alive mask (hot_eligible takes them without RF factors only) run a loss build
 #define PP_HOT_CASE(A)                                                                                  \
     case A:                                                                                             \
-        if (loss) {                                                                                     \
+        if (loss && p.mesh) {                                                                           \
+            PP_HOT_MESH_LAUNCH(A)                                                                       \
+        } else if (loss) {                                                                              \
             PP_HOT_LAUNCH(A, false, true)                                                               \
         } else if (rf) {                                                                                \
             PP_HOT_LAUNCH(A, true, false)                                                               \
