@@ -50,6 +50,30 @@ def test_segments_match_brute_force(kind):
     assert not np.any(got[:50] & ~want[:50])
 
 
+def test_listed_and_crowded_cells_match_brute_force():
+    """Short segments are decided by the level-0 triangle lists where every cell they cover has one, and by
+    the hierarchy walk elsewhere.  Here 160 large triangles crowd the left half (about a quarter of its cells
+    are overlapped by more than the 32 triangles a list holds, so segments there cover listed cells, cells
+    without a list, or both), small triangles fill the right half, and short random segments land on either
+    side and across the middle: the device's hit masks equal the brute force."""
+    rng = np.random.default_rng(63)
+    nb = 160
+    big = np.column_stack([rng.uniform(-1.0, -0.1, (3 * nb, 1)), rng.uniform(-1.0, 1.0, (3 * nb, 2))])
+    vs, ts = _soup(rng, 2000)
+    vs = vs * np.array([0.45, 1.0, 1.0]) + np.array([0.55, 0.0, 0.0])
+    v = np.vstack([big, vs])
+    t = np.vstack([np.arange(3 * nb).reshape(nb, 3), ts + 3 * nb]).astype(np.int32)
+    m = TriangleElectrodes(v, t)
+    n = 20000
+    r0 = rng.uniform(-1.1, 1.1, (n, 3))
+    r1 = r0 + 0.05 * rng.standard_normal((n, 3))
+    got = m.segment_intersects_surface(r0, r1)["hit_mask"]
+    want = segment_hits(m.vertices, m.triangles, r0, r1)
+    np.testing.assert_array_equal(got, want)
+    left, right = r0[:, 0] < -0.2, r0[:, 0] > 0.2
+    assert want[left].mean() > 0.05 and want[right].mean() > 0.05
+
+
 def test_aimed_segments_over_many_candidate_cells_match_brute_force():
     """A 12^3 lattice of small triangles (circumradius 0.01, normal (1,1,1)) keeps each candidate cell's
     subtree narrow, so a cell left out of a segment's candidate set loses the hit.  Segments are aimed at
