@@ -594,6 +594,19 @@ DEV double2 as_d2(float4 v) {
     return d;
 }
 
+// Wave issue priority of the hot gathers: raised from the cell lookup until the cell data is staged (or
+// the gather is done), so that wavefronts about to issue their staging loads and scans go ahead of those in
+// their arithmetic and the loads of more wavefronts overlap (C2: 2.19e11 particle-steps/s against 2.11e11
+// at one level; priorities 1, 2 and 3, and dropping it after the evaluation instead, measured the same).
+DEV void gather_priority(bool raised) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (raised) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+#else
+    (void)raised;
+#endif
+}
+
 // Field multipliers of the FP32 hot gather: E = kE * e and B = kB * b.  kE, kB are the
 // field scales, times h = q/m dt / 2 for the Boris-family steps (step_mixed<ALG, true>).
 struct HotScale {
@@ -664,6 +677,7 @@ struct HotFE32 {
 
     // the six map components (E then B) at a point, unscaled
     DEV void gather(double x, double y, double z, float (&v)[6]) const {
+        gather_priority(true);
         double tx, ty, tz;
         bool ok;
         uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
@@ -701,6 +715,7 @@ struct HotFE32 {
             const int ns = distinct_cells(cell, myslot);
             if (ns > 0) {
                 stage_cells<12>(wv, stage_list(), rs, 192u, ns, cell, myslot, lanes);
+                gather_priority(false);
                 const float4* q = wv + __umul24(myslot, (uint32_t)kPitch);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) v[c] = ceval(q[2 * c], q[2 * c + 1], wx, wy, wz);
@@ -720,6 +735,7 @@ struct HotFE32 {
             for (int c = 0; c < 6; ++c)
                 v[c] = ceval(ld4f(rs, base + c * 32u), ld4f(rs, base + c * 32u + 16u), wx, wy, wz);
         }
+        gather_priority(false);
     }
 };
 
@@ -736,6 +752,7 @@ struct HotFE64 {
     mutable uint32_t last_slot = 0;
     mutable bool staged = false;
     DEV void operator()(int64_t, double x, double y, double z, int64_t gs, int stage, d3& E, d3& B) const {
+        gather_priority(true);
         double tx, ty, tz;
         bool ok;
         uint32_t cell = hot_locate<PERIODIC>(m, x, y, z, tx, ty, tz, ok);
@@ -756,6 +773,7 @@ struct HotFE64 {
             const int ns = distinct_cells(cell, slot);
             if (ns > 0) {
                 stage_cells<24>(wv, stage_list(), rs, 384u, ns, cell, slot, lanes);
+                gather_priority(false);
                 done = true;
                 if constexpr (REUSE) {
                     last_cell = cell;
@@ -781,6 +799,7 @@ struct HotFE64 {
                 v[c] = cevald(ld2d(rs, o), ld2d(rs, o + 16u), ld2d(rs, o + 32u), ld2d(rs, o + 48u), tx, ty, tz);
             }
         }
+        gather_priority(false);
         double sE = m.sE, sB = m.sB;
         if constexpr (RF) {
             if (rf.e != PP_RF_NONE) sE *= hot_rf(rf, rf.e, gs, stage);
