@@ -596,8 +596,9 @@ DEV double2 as_d2(float4 v) {
 
 // Wave issue priority of the hot gathers: raised from the cell lookup until the cell data is staged (or
 // the gather is done), so that wavefronts about to issue their staging loads and scans go ahead of those in
-// their arithmetic and the loads of more wavefronts overlap (C2: 2.19e11 particle-steps/s against 2.11e11
-// at one level; priorities 1, 2 and 3, and dropping it after the evaluation instead, measured the same).
+// their arithmetic and the loads of more wavefronts overlap.  Measured on C2: 2.19e11 particle-steps/s
+// against 2.11e11 with every wavefront at the same priority; priorities 1, 2 and 3, and dropping it only
+// after the evaluation, ran alike.
 DEV void gather_priority(bool raised) {
 #if defined(__HIP_DEVICE_COMPILE__)
     if (raised) __builtin_amdgcn_s_setprio(1);
